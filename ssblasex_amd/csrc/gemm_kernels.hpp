@@ -1,0 +1,501 @@
+// gemm_kernels.hpp -- the two kernel families of the batched GEMM.
+//
+// tile_gemm: LDS-tiled persistent kernel for instances of 64x64 and up.
+//   One workgroup = WAVES_M x WAVES_N waves computes one BM x BN output
+//   tile of one instance per iteration of its persistent loop over
+//   (instance, tile) work ids.  A and B tiles are staged global -> registers
+//   -> LDS with two LDS buffers (the next k-block's global loads are in
+//   flight while the current block's MFMAs run).  The staging absorbs the
+//   OP_N / OP_T layouts: a K-contiguous operand is copied 16 bytes at a time,
+//   an M/N-contiguous operand is loaded as small 2-D blocks and transposed
+//   in registers so that LDS always holds [row][k] with k contiguous.
+//   Epilogue: C = alpha*acc + beta*C fused, C not read when beta == 0.
+//
+// direct_gemm: one wave per WTM x WTN block, operands loaded straight from
+//   global memory into MFMA registers (no LDS, no barriers).  For small
+//   instances (both m and n at most 64) where a tile would be mostly empty.
+//
+// Operand view: op(A) element (i, k) sits at A[i*sra + k*ska] and op(B)
+// element (k, j) at B[j*srb + k*skb] (element strides).  OP_T for A and
+// OP_N for B make k the contiguous index (ska == 1 / skb == 1).
+#pragma once
+#include "device_common.hpp"
+
+namespace ssb {
+
+struct KParams {
+    const void* const* A;
+    const void* const* B;
+    void* const* C;
+    long lda, ldb, ldc;
+    int m, n, k;
+    int batch;
+    int tiles_m, tiles_n;  // per instance
+    long total;            // batch * tiles_m * tiles_n
+    Scalars s;
+    int beta0;             // beta == 0: C is write-only
+    int ld_vec_a, ld_vec_b, ld_vec_c;  // leading dimensions allow 16-byte vectors
+};
+
+// ---------------------------------------------------------------------------
+// Element access helpers
+// ---------------------------------------------------------------------------
+template <int E>
+struct Raw;
+template <> struct Raw<1> { using T = uint8_t; };
+template <> struct Raw<4> { using T = uint32_t; };
+template <> struct Raw<8> { using T = uint64_t; };
+
+template <int E>
+__device__ __forceinline__ typename Raw<E>::T load_elem(const char* base, long off_elems)
+{
+    return *reinterpret_cast<const typename Raw<E>::T*>(base + off_elems * E);
+}
+
+// ---------------------------------------------------------------------------
+// Stager: one operand tile (ROWS x BK input elements) global -> regs -> LDS.
+//
+// The tile is cut into units, one or more per thread.  A unit is read with
+// "words" (one global load each: 16 bytes, or 4 bytes for an M/N-contiguous
+// int8 operand) and written to LDS as [row][k] rows:
+//   K-contiguous operand:  unit = one word = WE consecutive k of one row.
+//   M/N-contiguous:        unit = RU rows x KU k, one word per k (the RU
+//                          rows are contiguous in memory); transposed in
+//                          registers before the LDS store.
+// ---------------------------------------------------------------------------
+template <class AR, bool KC, int ROWS, int BK, int NT>
+struct Stager {
+    static constexpr int E = AR::E;
+    static constexpr int WB = (!KC && E == 1) ? 4 : 16;  // bytes per word
+    static constexpr int WE = WB / E;                     // elements per word
+    static constexpr int WD = WB / 4;                     // dwords per word
+    static constexpr int RU = KC ? 1 : WE;                // rows per unit
+    static constexpr int KU = KC ? WE : (E == 1 ? 8 : WE); // k per unit
+    static constexpr int NW = KC ? 1 : KU;                // words per unit
+    static constexpr int RG = ROWS / RU;
+    static constexpr int KG = BK / KU;
+    static constexpr int UNITS = RG * KG;
+    static constexpr int UPT = (UNITS + NT - 1) / NT;
+    static_assert(BK % KU == 0, "BK must hold whole units");
+    static_assert(ROWS % RU == 0, "rows must hold whole units");
+
+    uint32_t raw[UPT][NW * WD];
+
+    __device__ __forceinline__ static void unit_coords(int u, int& r0, int& k0)
+    {
+        if constexpr (KC) { k0 = (u % KG) * KU; r0 = u / KG; }
+        else { r0 = (u % RG) * RU; k0 = (u / RG) * KU; }
+    }
+    __device__ __forceinline__ static bool active(int u) { return (UNITS % NT) == 0 || u < UNITS; }
+
+    // address (bytes from base) of word w of a unit
+    __device__ __forceinline__ static long word_off(int r0, int k0, int w, long sr, long sk)
+    {
+        if constexpr (KC) return ((long)r0 * sr + k0) * E;
+        else return ((long)r0 + (long)(k0 + w) * sk) * E;
+    }
+
+    __device__ __forceinline__ void put_word(int x, int w, const char* src)
+    {
+        if constexpr (WB == 16) {
+            u32x4 v = *reinterpret_cast<const u32x4*>(src);
+#pragma unroll
+            for (int d = 0; d < 4; ++d) raw[x][w * 4 + d] = v[d];
+        } else {
+            raw[x][w] = *reinterpret_cast<const uint32_t*>(src);
+        }
+    }
+
+    // Every row and k of this tile step in range, words aligned.
+    __device__ __forceinline__ void load_fast(const char* base, long sr, long sk, int tid)
+    {
+#pragma unroll
+        for (int x = 0; x < UPT; ++x) {
+            const int u = tid + x * NT;
+            if (!active(u)) break;
+            int r0, k0;
+            unit_coords(u, r0, k0);
+#pragma unroll
+            for (int w = 0; w < NW; ++w) put_word(x, w, base + word_off(r0, k0, w, sr, sk));
+        }
+    }
+
+    // Zero outside rows_valid x k_valid; a word is read with one vector load
+    // when it is whole and `vec`, else element by element.
+    __device__ __forceinline__ void load_guarded(const char* base, long sr, long sk, int rows_valid,
+                                                 int k_valid, bool vec, int tid)
+    {
+        using RT = typename Raw<E>::T;
+#pragma unroll
+        for (int x = 0; x < UPT; ++x) {
+            const int u = tid + x * NT;
+            if (!active(u)) break;
+            int r0, k0;
+            unit_coords(u, r0, k0);
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                bool whole;
+                if constexpr (KC) whole = r0 < rows_valid && k0 + WE <= k_valid;
+                else whole = r0 + RU <= rows_valid && k0 + w < k_valid;
+                if (vec && whole) {
+                    put_word(x, w, base + word_off(r0, k0, w, sr, sk));
+                } else {
+                    RT vals[WE];
+#pragma unroll
+                    for (int e = 0; e < WE; ++e) {
+                        bool ok;
+                        long off;
+                        if constexpr (KC) {
+                            ok = r0 < rows_valid && k0 + e < k_valid;
+                            off = (long)r0 * sr + (k0 + e);
+                        } else {
+                            ok = r0 + e < rows_valid && k0 + w < k_valid;
+                            off = (long)(r0 + e) + (long)(k0 + w) * sk;
+                        }
+                        vals[e] = ok ? load_elem<E>(base, off) : RT(0);
+                    }
+                    uint32_t d[WD];
+                    __builtin_memcpy(d, vals, WB);
+#pragma unroll
+                    for (int i = 0; i < WD; ++i) raw[x][w * WD + i] = d[i];
+                }
+            }
+        }
+    }
+
+    // Write the staged units to LDS planes ([row][k], pitch rowb bytes).
+    __device__ __forceinline__ void store(char* lds, int plane_bytes, int rowb, int tid) const
+    {
+#pragma unroll
+        for (int x = 0; x < UPT; ++x) {
+            const int u = tid + x * NT;
+            if (!active(u)) break;
+            int r0, k0;
+            unit_coords(u, r0, k0);
+            // rows[i]: the KU*E bytes of row r0+i (16 bytes, or 8 for int8 M/N-contiguous)
+            u32x4 rows[RU];
+            if constexpr (KC) {
+                rows[0] = u32x4{raw[x][0], raw[x][1], raw[x][2], raw[x][3]};
+            } else if constexpr (E == 1) {
+                uint32_t o0[4], o1[4];
+                transpose4x4_bytes(&raw[x][0], o0);  // k0..k0+3
+                transpose4x4_bytes(&raw[x][4], o1);  // k0+4..k0+7
+#pragma unroll
+                for (int i = 0; i < 4; ++i) rows[i] = u32x4{o0[i], o1[i], 0u, 0u};
+            } else if constexpr (E == 4) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    rows[i] = u32x4{raw[x][i], raw[x][4 + i], raw[x][8 + i], raw[x][12 + i]};
+            } else {  // E == 8: word w = rows r0, r0+1 at k0+w
+                rows[0] = u32x4{raw[x][0], raw[x][1], raw[x][4], raw[x][5]};
+                rows[1] = u32x4{raw[x][2], raw[x][3], raw[x][6], raw[x][7]};
+            }
+            const int kbyte = k0 * AR::LDS_E;
+#pragma unroll
+            for (int i = 0; i < RU; ++i) {
+                char* dst = lds + (r0 + i) * rowb + kbyte;
+                if constexpr (!KC && E == 1) {
+                    *reinterpret_cast<u32x2*>(dst) = u32x2{rows[i][0], rows[i][1]};
+                } else if constexpr (AR::LDS_E == E) {
+                    *reinterpret_cast<u32x4*>(dst) = rows[i];
+                } else {
+                    u32x4 planes[AR::NPLANE];
+                    AR::to_planes(rows[i], planes);
+#pragma unroll
+                    for (int pl = 0; pl < AR::NPLANE; ++pl)
+                        *reinterpret_cast<u32x2*>(dst + pl * plane_bytes) = u32x2{planes[pl][0], planes[pl][1]};
+                }
+            }
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Epilogue for one 16x16 accumulator fragment.
+//   r0: first of the lane's 4 consecutive rows, c: the lane's column.
+// ---------------------------------------------------------------------------
+template <class AR>
+__device__ __forceinline__ void store_frag(typename AR::Out* C, long ldc, int r0, int c,
+                                           const typename AR::Acc& acc, const KParams& p,
+                                           bool fast, int m, int n)
+{
+    using Out = typename AR::Out;
+    const bool beta0 = p.beta0 != 0;
+    if (fast) {
+        Out* dst = C + (long)c * ldc + r0;
+        if constexpr (sizeof(Out) == 4) {
+            // Elements go through plain arrays and memcpy: hipcc 7.2 folds
+            // __builtin_bit_cast of ext-vector elements in this unrolled
+            // loop into one reused dword.
+            Out old[4] = {Out(0), Out(0), Out(0), Out(0)};
+            if (!beta0) {
+                u32x4 t = *reinterpret_cast<const u32x4*>(dst);
+                uint32_t w[4] = {t.x, t.y, t.z, t.w};
+                __builtin_memcpy(old, w, 16);
+            }
+            Out res[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) res[v] = epi(acc[v], old[v], p.s, beta0);
+            uint32_t w[4];
+            __builtin_memcpy(w, res, 16);
+            *reinterpret_cast<u32x4*>(dst) = u32x4{w[0], w[1], w[2], w[3]};
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                f64x2 old = {0.0, 0.0};
+                if (!beta0) old = *reinterpret_cast<const f64x2*>(dst + 2 * h);
+                f64x2 res;
+                res[0] = epi((double)acc[2 * h], old[0], p.s, beta0);
+                res[1] = epi((double)acc[2 * h + 1], old[1], p.s, beta0);
+                *reinterpret_cast<f64x2*>(dst + 2 * h) = res;
+            }
+        }
+    } else {
+        if (c >= n) return;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            int r = r0 + v;
+            if (r < m) {
+                Out* dst = C + (long)c * ldc + r;
+                Out o = beta0 ? Out(0) : *dst;
+                *dst = epi(acc[v], o, p.s, beta0);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS-tiled persistent kernel
+// ---------------------------------------------------------------------------
+template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N>
+struct TileShape {
+    static constexpr int NT = 64 * WAVES_M * WAVES_N;
+    static constexpr int ROWB = BK * AR::LDS_E + 16;  // LDS row pitch (16 B pad)
+    static constexpr int A_PLANE = BM * ROWB;
+    static constexpr int B_PLANE = BN * ROWB;
+    static constexpr int BUF = AR::NPLANE * (A_PLANE + B_PLANE);
+    static constexpr int LDS_BYTES = 2 * BUF;
+    static constexpr int WTM = BM / WAVES_M, WTN = BN / WAVES_N;
+    static constexpr int FM = WTM / 16, FN = WTN / 16;
+    static_assert(WTM % 16 == 0 && WTN % 16 == 0, "wave tile must be whole 16x16 fragments");
+    static_assert(BK % AR::KSTEP == 0, "BK must hold whole MFMA k-steps");
+};
+
+template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N>
+__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
+tile_gemm(const KParams p)
+{
+    using S = TileShape<AR, AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N>;
+    using Out = typename AR::Out;
+    constexpr int NT = S::NT;
+    constexpr int E = AR::E;
+    constexpr int KC_L = 16 / AR::LDS_E;  // LDS elements per 16-byte fragment chunk
+
+    __shared__ __attribute__((aligned(16))) char smem[S::LDS_BYTES];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+
+    // element strides of the operand views
+    const long sra = AKC ? p.lda : 1, ska = AKC ? 1 : p.lda;
+    const long srb = BKC ? p.ldb : 1, skb = BKC ? 1 : p.ldb;
+    const int tiles_per_inst = p.tiles_m * p.tiles_n;
+    const int nk = (p.k + BK - 1) / BK;
+
+    Stager<AR, AKC, BM, BK, NT> sa;
+    Stager<AR, BKC, BN, BK, NT> sb;
+
+    for (long t = blockIdx.x; t < p.total; t += gridDim.x) {
+        const long w = xcd_remap(t, p.total);
+        const int inst = (int)(w / tiles_per_inst);
+        const int rem = (int)(w % tiles_per_inst);
+        const int i0 = (rem % p.tiles_m) * BM;
+        const int j0 = (rem / p.tiles_m) * BN;
+        const char* A = static_cast<const char*>(p.A[inst]);
+        const char* B = static_cast<const char*>(p.B[inst]);
+        Out* C = static_cast<Out*>(p.C[inst]);
+        const int rows_a = min(BM, p.m - i0);
+        const int rows_b = min(BN, p.n - j0);
+        using SA = Stager<AR, AKC, BM, BK, NT>;
+        using SB = Stager<AR, BKC, BN, BK, NT>;
+        const bool a_vec = ((p.lda * E) % SA::WB) == 0 && (reinterpret_cast<uintptr_t>(A) % SA::WB) == 0;
+        const bool b_vec = ((p.ldb * E) % SB::WB) == 0 && (reinterpret_cast<uintptr_t>(B) % SB::WB) == 0;
+        const bool a_fast = a_vec && rows_a == BM;
+        const bool b_fast = b_vec && rows_b == BN;
+        // tile origin (row i0 / j0, k 0) in bytes
+        const char* Ab = A + (long)i0 * sra * E;
+        const char* Bb = B + (long)j0 * srb * E;
+
+        typename AR::Acc acc[S::FM][S::FN];
+#pragma unroll
+        for (int fm = 0; fm < S::FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < S::FN; ++fn) acc[fm][fn] = typename AR::Acc{};
+
+        auto load_step = [&](int kt) {
+            const int k0 = kt * BK;
+            const int kv = p.k - k0;
+            const char* ak = Ab + (long)k0 * ska * E;
+            const char* bk = Bb + (long)k0 * skb * E;
+            if (a_fast && kv >= BK) sa.load_fast(ak, sra, ska, tid);
+            else sa.load_guarded(ak, sra, ska, rows_a, kv, a_vec, tid);
+            if (b_fast && kv >= BK) sb.load_fast(bk, srb, skb, tid);
+            else sb.load_guarded(bk, srb, skb, rows_b, kv, b_vec, tid);
+        };
+        auto store_step = [&](int buf) {
+            char* base = smem + buf * S::BUF;
+            sa.store(base, S::A_PLANE, S::ROWB, tid);
+            sb.store(base + AR::NPLANE * S::A_PLANE, S::B_PLANE, S::ROWB, tid);
+        };
+
+        load_step(0);
+        __syncthreads();  // previous tile's LDS reads are complete
+        store_step(0);
+        __syncthreads();
+
+        for (int kt = 0; kt < nk; ++kt) {
+            const int buf = kt & 1;
+            if (kt + 1 < nk) load_step(kt + 1);
+            const char* la = smem + buf * S::BUF;
+            const char* lb = la + AR::NPLANE * S::A_PLANE;
+#pragma unroll
+            for (int ks = 0; ks < BK / AR::KSTEP; ++ks) {
+                const int kbyte = (ks * AR::KSTEP + (lane >> 4) * KC_L) * AR::LDS_E;
+                u32x4 af[S::FM][AR::NPLANE], bfg[S::FN][AR::NPLANE];
+#pragma unroll
+                for (int fm = 0; fm < S::FM; ++fm) {
+                    const int row = wm * S::WTM + fm * 16 + frag_row<AR>(lane & 15);
+#pragma unroll
+                    for (int pl = 0; pl < AR::NPLANE; ++pl)
+                        af[fm][pl] = *reinterpret_cast<const u32x4*>(la + pl * S::A_PLANE + row * S::ROWB + kbyte);
+                }
+#pragma unroll
+                for (int fn = 0; fn < S::FN; ++fn) {
+                    const int row = wn * S::WTN + fn * 16 + (lane & 15);
+#pragma unroll
+                    for (int pl = 0; pl < AR::NPLANE; ++pl)
+                        bfg[fn][pl] = *reinterpret_cast<const u32x4*>(lb + pl * S::B_PLANE + row * S::ROWB + kbyte);
+                }
+#pragma unroll
+                for (int fm = 0; fm < S::FM; ++fm)
+#pragma unroll
+                    for (int fn = 0; fn < S::FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
+            }
+            if (kt + 1 < nk) store_step(buf ^ 1);
+            __syncthreads();
+        }
+
+        // fused alpha/beta epilogue
+        const bool c_fast = rows_a == BM && rows_b == BN && p.ld_vec_c && aligned16(C);
+#pragma unroll
+        for (int fm = 0; fm < S::FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < S::FN; ++fn) {
+                const int r0 = i0 + wm * S::WTM + fm * 16 + 4 * (lane >> 4);
+                const int c = j0 + wn * S::WTN + fn * 16 + (lane & 15);
+                store_frag<AR>(C, p.ldc, r0, c, acc[fm][fn], p, c_fast, p.m, p.n);
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Direct (LDS-free) kernel: one wave per WTM x WTN block.
+// ---------------------------------------------------------------------------
+
+// Load the lane's operand fragment for one MFMA k-step straight from global:
+// row `r` of the operand view, input elements k = kb + g*KL + s, s < KL,
+// KL = KSTEP/4.  Returns NPLANE 16-byte plane chunks.
+template <class AR, bool KC>
+__device__ __forceinline__ void direct_frag(const char* base, long sr, long sk, int r, int rows,
+                                            int kb, int kvalid, bool vec, u32x4* out)
+{
+    constexpr int E = AR::E;
+    constexpr int KL = AR::KSTEP / 4;          // input elements per lane per k-step
+    constexpr int NIN = KL * E / 16;           // 16-byte input words (1, or 2 for bf16)
+    using RT = typename Raw<E>::T;
+    u32x4 in[NIN];
+    const int g = threadIdx.x & 63;
+    const int k0 = kb + (g >> 4) * KL;
+    if (KC && vec && r < rows && k0 + KL <= kvalid) {
+#pragma unroll
+        for (int w = 0; w < NIN; ++w)
+            in[w] = *reinterpret_cast<const u32x4*>(base + ((long)r * sr + k0) * E + 16 * w);
+    } else {
+        RT vals[KL];
+#pragma unroll
+        for (int s = 0; s < KL; ++s) {
+            bool ok = (r < rows) && (k0 + s < kvalid);
+            vals[s] = ok ? load_elem<E>(base, (long)r * sr + (long)(k0 + s) * sk) : RT(0);
+        }
+#pragma unroll
+        for (int w = 0; w < NIN; ++w) __builtin_memcpy(&in[w], &vals[w * 16 / E], 16);
+    }
+    if constexpr (AR::LDS_E == E) {
+        out[0] = in[0];
+    } else {
+        // bf16 split: two 16-byte input words -> one 16-byte chunk per plane
+        u32x4 lo[AR::NPLANE], hi[AR::NPLANE];
+        AR::to_planes(in[0], lo);
+        AR::to_planes(in[NIN - 1], hi);
+#pragma unroll
+        for (int pl = 0; pl < AR::NPLANE; ++pl) out[pl] = u32x4{lo[pl][0], lo[pl][1], hi[pl][0], hi[pl][1]};
+    }
+}
+
+template <class AR, bool AKC, bool BKC, int WTM, int WTN>
+__global__ void __launch_bounds__(256)
+direct_gemm(const KParams p)
+{
+    using Out = typename AR::Out;
+    constexpr int FM = WTM / 16, FN = WTN / 16;
+    const int lane = threadIdx.x & 63;
+    const long sra = AKC ? p.lda : 1, ska = AKC ? 1 : p.lda;
+    const long srb = BKC ? p.ldb : 1, skb = BKC ? 1 : p.ldb;
+    const int per_inst = p.tiles_m * p.tiles_n;
+    const long nwaves = (long)gridDim.x * 4;
+
+    for (long t = (long)blockIdx.x * 4 + (threadIdx.x >> 6); t < p.total; t += nwaves) {
+        const int inst = (int)(t / per_inst);
+        const int rem = (int)(t % per_inst);
+        const int i0 = (rem % p.tiles_m) * WTM;
+        const int j0 = (rem / p.tiles_m) * WTN;
+        const char* A = static_cast<const char*>(p.A[inst]);
+        const char* B = static_cast<const char*>(p.B[inst]);
+        Out* C = static_cast<Out*>(p.C[inst]);
+        const bool a_vec = p.ld_vec_a && aligned16(A);
+        const bool b_vec = p.ld_vec_b && aligned16(B);
+
+        typename AR::Acc acc[FM][FN];
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = typename AR::Acc{};
+
+        for (int kb = 0; kb < p.k; kb += AR::KSTEP) {
+            u32x4 af[FM][AR::NPLANE], bfg[FN][AR::NPLANE];
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm)
+                direct_frag<AR, AKC>(A, sra, ska, i0 + fm * 16 + frag_row<AR>(lane & 15), p.m, kb, p.k, a_vec,
+                                     af[fm]);
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn)
+                direct_frag<AR, BKC>(B, srb, skb, j0 + fn * 16 + (lane & 15), p.n, kb, p.k, b_vec, bfg[fn]);
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
+        }
+
+        const bool c_fast = i0 + WTM <= p.m && j0 + WTN <= p.n && p.ld_vec_c && aligned16(C);
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn)
+                store_frag<AR>(C, p.ldc, i0 + fm * 16 + 4 * (lane >> 4), j0 + fn * 16 + (lane & 15),
+                               acc[fm][fn], p, c_fast, p.m, p.n);
+    }
+}
+
+}  // namespace ssb

@@ -1,0 +1,65 @@
+// launch.hpp -- internal interface between the host API layer (api.cpp) and
+// the kernel translation units.  Plain C++ types only, so api.cpp compiles
+// with the host compiler.
+#pragma once
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+namespace ssb {
+
+enum ArithKind : int {
+    K_I8 = 0,
+    K_F32 = 1,
+    K_F64 = 2,
+    K_BF16X3 = 3,
+    K_BF16X6 = 4,
+};
+
+// Tile configurations.  DIRECT_*: one wave per block, no LDS.  TILE_*: LDS
+// double-buffered workgroup tiles.  Per-type k-depths are fixed per config.
+enum TileConfig : int {
+    CFG_DIRECT_16 = 0,   // wave per 16x16
+    CFG_DIRECT_32 = 1,   // wave per 32x32
+    CFG_TILE_64 = 2,     // 64x64 tile, 4 waves of 32x32
+    CFG_TILE_128 = 3,    // 128x128 tile, 4 waves of 64x64
+    CFG_TILE_128x64 = 4, // 128x64 tile (M x N), 4 waves of 64x32
+    CFG_TILE_64x128 = 5, // 64x128 tile, 4 waves of 32x64
+    CFG_COUNT = 6
+};
+
+const char* config_name(int cfg);
+void config_tile(int cfg, int* tm, int* tn);
+
+struct GemmProblem {
+    int arith;             // ArithKind
+    bool ta, tb;           // op(A) == A^T, op(B) == B^T
+    int m, n, k;
+    long lda, ldb, ldc;
+    int batch;
+    const void* const* A;  // device-accessible arrays of device pointers
+    const void* const* B;
+    void* const* C;
+    float af, bf;          // scalars in every representation
+    double ad, bd;
+    int32_t ai, bi;
+    bool beta0;
+};
+
+// Enqueue one batched GEMM with the given tile config on `stream`.
+hipError_t launch_gemm(const GemmProblem& g, int cfg, hipStream_t stream);
+
+// Default tile config for a problem (size-bucket heuristic, overridable by
+// the autotune table).
+int default_config(const GemmProblem& g);
+
+// Per-arith entry points (one translation unit each).
+hipError_t launch_i8(const GemmProblem& g, int cfg, hipStream_t stream);
+hipError_t launch_f32(const GemmProblem& g, int cfg, hipStream_t stream);
+hipError_t launch_f64(const GemmProblem& g, int cfg, hipStream_t stream);
+hipError_t launch_bf16x3(const GemmProblem& g, int cfg, hipStream_t stream);
+hipError_t launch_bf16x6(const GemmProblem& g, int cfg, hipStream_t stream);
+
+// Number of compute units of the current device (cached per device).
+int device_cu_count();
+
+}  // namespace ssb
