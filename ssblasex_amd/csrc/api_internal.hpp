@@ -27,5 +27,10 @@ int tuned_config(const GemmProblem& g);
 // Ozaki-scheme fp64 (slices == 0: pick automatically).  Sets *st to a
 // non-SUCCESS status for failures that are not HIP errors.
 hipError_t launch_ozaki(const GemmProblem& g, int slices, hipStream_t stream, ssblasStatus_t* st);
+// Device side of the Ozaki form (ozaki.hip); ws_get returns a device buffer
+// of at least the requested size owned by the caller's stream.
+hipError_t ozaki_run(const GemmProblem& g, int s, void* (*ws_get)(size_t, hipError_t*), hipStream_t st);
+// Slice count used when the compute type does not name one.
+constexpr int OZAKI_DEFAULT_SLICES = 9;
 
 }  // namespace ssb

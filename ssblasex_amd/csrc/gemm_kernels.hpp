@@ -216,11 +216,10 @@ struct Stager {
 // ---------------------------------------------------------------------------
 template <class AR>
 __device__ __forceinline__ void store_frag(typename AR::Out* C, long ldc, int r0, int c,
-                                           const typename AR::Acc& acc, const KParams& p,
-                                           bool fast, int m, int n)
+                                           const typename AR::Acc& acc, const Scalars& sc,
+                                           bool beta0, bool fast, int m, int n)
 {
     using Out = typename AR::Out;
-    const bool beta0 = p.beta0 != 0;
     if (fast) {
         Out* dst = C + (long)c * ldc + r0;
         if constexpr (sizeof(Out) == 4) {
@@ -235,7 +234,7 @@ __device__ __forceinline__ void store_frag(typename AR::Out* C, long ldc, int r0
             }
             Out res[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) res[v] = epi(acc[v], old[v], p.s, beta0);
+            for (int v = 0; v < 4; ++v) res[v] = epi(acc[v], old[v], sc, beta0);
             uint32_t w[4];
             __builtin_memcpy(w, res, 16);
             *reinterpret_cast<u32x4*>(dst) = u32x4{w[0], w[1], w[2], w[3]};
@@ -245,8 +244,8 @@ __device__ __forceinline__ void store_frag(typename AR::Out* C, long ldc, int r0
                 f64x2 old = {0.0, 0.0};
                 if (!beta0) old = *reinterpret_cast<const f64x2*>(dst + 2 * h);
                 f64x2 res;
-                res[0] = epi((double)acc[2 * h], old[0], p.s, beta0);
-                res[1] = epi((double)acc[2 * h + 1], old[1], p.s, beta0);
+                res[0] = epi((double)acc[2 * h], old[0], sc, beta0);
+                res[1] = epi((double)acc[2 * h + 1], old[1], sc, beta0);
                 *reinterpret_cast<f64x2*>(dst + 2 * h) = res;
             }
         }
@@ -258,15 +257,26 @@ __device__ __forceinline__ void store_frag(typename AR::Out* C, long ldc, int r0
             if (r < m) {
                 Out* dst = C + (long)c * ldc + r;
                 Out o = beta0 ? Out(0) : *dst;
-                *dst = epi(acc[v], o, p.s, beta0);
+                *dst = epi(acc[v], o, sc, beta0);
             }
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// LDS-tiled persistent kernel
+// One output tile of one instance: what a work item hands to a tile routine.
 // ---------------------------------------------------------------------------
+struct TileJob {
+    const char* A;  // instance base pointers
+    const char* B;
+    void* C;
+    long lda, ldb, ldc;
+    int m, n, k;
+    int i0, j0;     // tile origin in C
+    Scalars s;
+    int beta0;
+};
+
 template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N>
 struct TileShape {
     static constexpr int NT = 64 * WAVES_M * WAVES_N;
@@ -281,17 +291,19 @@ struct TileShape {
     static_assert(BK % AR::KSTEP == 0, "BK must hold whole MFMA k-steps");
 };
 
+// LDS-staged workgroup tile: BM x BN of C, all threads of the workgroup.
+// smem must hold TileShape::LDS_BYTES; the caller's previous use of it must
+// be finished by every thread (the routine starts with a barrier).
 template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N>
-__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
-tile_gemm(const KParams p)
+__device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
 {
     using S = TileShape<AR, AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N>;
     using Out = typename AR::Out;
     constexpr int NT = S::NT;
     constexpr int E = AR::E;
     constexpr int KC_L = 16 / AR::LDS_E;  // LDS elements per 16-byte fragment chunk
-
-    __shared__ __attribute__((aligned(16))) char smem[S::LDS_BYTES];
+    using SA = Stager<AR, AKC, BM, BK, NT>;
+    using SB = Stager<AR, BKC, BN, BK, NT>;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -299,109 +311,131 @@ tile_gemm(const KParams p)
     const int wm = wave % WAVES_M, wn = wave / WAVES_M;
 
     // element strides of the operand views
-    const long sra = AKC ? p.lda : 1, ska = AKC ? 1 : p.lda;
-    const long srb = BKC ? p.ldb : 1, skb = BKC ? 1 : p.ldb;
+    const long sra = AKC ? j.lda : 1, ska = AKC ? 1 : j.lda;
+    const long srb = BKC ? j.ldb : 1, skb = BKC ? 1 : j.ldb;
+    const int nk = (j.k + BK - 1) / BK;
+    const int rows_a = min(BM, j.m - j.i0);
+    const int rows_b = min(BN, j.n - j.j0);
+    const bool a_vec = ((j.lda * E) % SA::WB) == 0 && (reinterpret_cast<uintptr_t>(j.A) % SA::WB) == 0;
+    const bool b_vec = ((j.ldb * E) % SB::WB) == 0 && (reinterpret_cast<uintptr_t>(j.B) % SB::WB) == 0;
+    const bool a_fast = a_vec && rows_a == BM;
+    const bool b_fast = b_vec && rows_b == BN;
+    // tile origin (row i0 / j0, k 0) in bytes
+    const char* Ab = j.A + (long)j.i0 * sra * E;
+    const char* Bb = j.B + (long)j.j0 * srb * E;
+
+    SA sa;
+    SB sb;
+    typename AR::Acc acc[S::FM][S::FN];
+#pragma unroll
+    for (int fm = 0; fm < S::FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < S::FN; ++fn) acc[fm][fn] = typename AR::Acc{};
+
+    auto load_step = [&](int kt) {
+        const int k0 = kt * BK;
+        const int kv = j.k - k0;
+        const char* ak = Ab + (long)k0 * ska * E;
+        const char* bk = Bb + (long)k0 * skb * E;
+        if (a_fast && kv >= BK) sa.load_fast(ak, sra, ska, tid);
+        else sa.load_guarded(ak, sra, ska, rows_a, kv, a_vec, tid);
+        if (b_fast && kv >= BK) sb.load_fast(bk, srb, skb, tid);
+        else sb.load_guarded(bk, srb, skb, rows_b, kv, b_vec, tid);
+    };
+    auto store_step = [&](int buf) {
+        char* base = smem + buf * S::BUF;
+        sa.store(base, S::A_PLANE, S::ROWB, tid);
+        sb.store(base + AR::NPLANE * S::A_PLANE, S::B_PLANE, S::ROWB, tid);
+    };
+
+    load_step(0);
+    __syncthreads();  // previous user of smem is done
+    store_step(0);
+    __syncthreads();
+
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) load_step(kt + 1);
+        const char* la = smem + buf * S::BUF;
+        const char* lb = la + AR::NPLANE * S::A_PLANE;
+#pragma unroll
+        for (int ks = 0; ks < BK / AR::KSTEP; ++ks) {
+            const int kbyte = (ks * AR::KSTEP + (lane >> 4) * KC_L) * AR::LDS_E;
+            u32x4 af[S::FM][AR::NPLANE], bfg[S::FN][AR::NPLANE];
+#pragma unroll
+            for (int fm = 0; fm < S::FM; ++fm) {
+                const int row = wm * S::WTM + fm * 16 + frag_row<AR>(lane & 15);
+#pragma unroll
+                for (int pl = 0; pl < AR::NPLANE; ++pl)
+                    af[fm][pl] = *reinterpret_cast<const u32x4*>(la + pl * S::A_PLANE + row * S::ROWB + kbyte);
+            }
+#pragma unroll
+            for (int fn = 0; fn < S::FN; ++fn) {
+                const int row = wn * S::WTN + fn * 16 + (lane & 15);
+#pragma unroll
+                for (int pl = 0; pl < AR::NPLANE; ++pl)
+                    bfg[fn][pl] = *reinterpret_cast<const u32x4*>(lb + pl * S::B_PLANE + row * S::ROWB + kbyte);
+            }
+#pragma unroll
+            for (int fm = 0; fm < S::FM; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < S::FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
+        }
+        if (kt + 1 < nk) store_step(buf ^ 1);
+        __syncthreads();
+    }
+
+    // fused alpha/beta epilogue
+    Out* C = static_cast<Out*>(j.C);
+    const bool c_fast = rows_a == BM && rows_b == BN && ((j.ldc * (long)sizeof(Out)) % 16) == 0 && aligned16(C);
+#pragma unroll
+    for (int fm = 0; fm < S::FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < S::FN; ++fn) {
+            const int r0 = j.i0 + wm * S::WTM + fm * 16 + 4 * (lane >> 4);
+            const int c = j.j0 + wn * S::WTN + fn * 16 + (lane & 15);
+            store_frag<AR>(C, j.ldc, r0, c, acc[fm][fn], j.s, j.beta0 != 0, c_fast, j.m, j.n);
+        }
+}
+
+__device__ __forceinline__ TileJob make_job(const KParams& p, int inst, int i0, int j0)
+{
+    TileJob j;
+    j.A = static_cast<const char*>(p.A[inst]);
+    j.B = static_cast<const char*>(p.B[inst]);
+    j.C = p.C[inst];
+    j.lda = p.lda;
+    j.ldb = p.ldb;
+    j.ldc = p.ldc;
+    j.m = p.m;
+    j.n = p.n;
+    j.k = p.k;
+    j.i0 = i0;
+    j.j0 = j0;
+    j.s = p.s;
+    j.beta0 = p.beta0;
+    return j;
+}
+
+// Batched persistent kernel: workgroups walk the (instance, tile) work ids.
+template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N>
+__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
+tile_gemm(const KParams p)
+{
+    using S = TileShape<AR, AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N>;
+    __shared__ __attribute__((aligned(16))) char smem[S::LDS_BYTES];
     const int tiles_per_inst = p.tiles_m * p.tiles_n;
-    const int nk = (p.k + BK - 1) / BK;
-
-    Stager<AR, AKC, BM, BK, NT> sa;
-    Stager<AR, BKC, BN, BK, NT> sb;
-
     for (long t = blockIdx.x; t < p.total; t += gridDim.x) {
         const long w = xcd_remap(t, p.total);
         const int inst = (int)(w / tiles_per_inst);
         const int rem = (int)(w % tiles_per_inst);
-        const int i0 = (rem % p.tiles_m) * BM;
-        const int j0 = (rem / p.tiles_m) * BN;
-        const char* A = static_cast<const char*>(p.A[inst]);
-        const char* B = static_cast<const char*>(p.B[inst]);
-        Out* C = static_cast<Out*>(p.C[inst]);
-        const int rows_a = min(BM, p.m - i0);
-        const int rows_b = min(BN, p.n - j0);
-        using SA = Stager<AR, AKC, BM, BK, NT>;
-        using SB = Stager<AR, BKC, BN, BK, NT>;
-        const bool a_vec = ((p.lda * E) % SA::WB) == 0 && (reinterpret_cast<uintptr_t>(A) % SA::WB) == 0;
-        const bool b_vec = ((p.ldb * E) % SB::WB) == 0 && (reinterpret_cast<uintptr_t>(B) % SB::WB) == 0;
-        const bool a_fast = a_vec && rows_a == BM;
-        const bool b_fast = b_vec && rows_b == BN;
-        // tile origin (row i0 / j0, k 0) in bytes
-        const char* Ab = A + (long)i0 * sra * E;
-        const char* Bb = B + (long)j0 * srb * E;
-
-        typename AR::Acc acc[S::FM][S::FN];
-#pragma unroll
-        for (int fm = 0; fm < S::FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < S::FN; ++fn) acc[fm][fn] = typename AR::Acc{};
-
-        auto load_step = [&](int kt) {
-            const int k0 = kt * BK;
-            const int kv = p.k - k0;
-            const char* ak = Ab + (long)k0 * ska * E;
-            const char* bk = Bb + (long)k0 * skb * E;
-            if (a_fast && kv >= BK) sa.load_fast(ak, sra, ska, tid);
-            else sa.load_guarded(ak, sra, ska, rows_a, kv, a_vec, tid);
-            if (b_fast && kv >= BK) sb.load_fast(bk, srb, skb, tid);
-            else sb.load_guarded(bk, srb, skb, rows_b, kv, b_vec, tid);
-        };
-        auto store_step = [&](int buf) {
-            char* base = smem + buf * S::BUF;
-            sa.store(base, S::A_PLANE, S::ROWB, tid);
-            sb.store(base + AR::NPLANE * S::A_PLANE, S::B_PLANE, S::ROWB, tid);
-        };
-
-        load_step(0);
-        __syncthreads();  // previous tile's LDS reads are complete
-        store_step(0);
-        __syncthreads();
-
-        for (int kt = 0; kt < nk; ++kt) {
-            const int buf = kt & 1;
-            if (kt + 1 < nk) load_step(kt + 1);
-            const char* la = smem + buf * S::BUF;
-            const char* lb = la + AR::NPLANE * S::A_PLANE;
-#pragma unroll
-            for (int ks = 0; ks < BK / AR::KSTEP; ++ks) {
-                const int kbyte = (ks * AR::KSTEP + (lane >> 4) * KC_L) * AR::LDS_E;
-                u32x4 af[S::FM][AR::NPLANE], bfg[S::FN][AR::NPLANE];
-#pragma unroll
-                for (int fm = 0; fm < S::FM; ++fm) {
-                    const int row = wm * S::WTM + fm * 16 + frag_row<AR>(lane & 15);
-#pragma unroll
-                    for (int pl = 0; pl < AR::NPLANE; ++pl)
-                        af[fm][pl] = *reinterpret_cast<const u32x4*>(la + pl * S::A_PLANE + row * S::ROWB + kbyte);
-                }
-#pragma unroll
-                for (int fn = 0; fn < S::FN; ++fn) {
-                    const int row = wn * S::WTN + fn * 16 + (lane & 15);
-#pragma unroll
-                    for (int pl = 0; pl < AR::NPLANE; ++pl)
-                        bfg[fn][pl] = *reinterpret_cast<const u32x4*>(lb + pl * S::B_PLANE + row * S::ROWB + kbyte);
-                }
-#pragma unroll
-                for (int fm = 0; fm < S::FM; ++fm)
-#pragma unroll
-                    for (int fn = 0; fn < S::FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
-            }
-            if (kt + 1 < nk) store_step(buf ^ 1);
-            __syncthreads();
-        }
-
-        // fused alpha/beta epilogue
-        const bool c_fast = rows_a == BM && rows_b == BN && p.ld_vec_c && aligned16(C);
-#pragma unroll
-        for (int fm = 0; fm < S::FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < S::FN; ++fn) {
-                const int r0 = i0 + wm * S::WTM + fm * 16 + 4 * (lane >> 4);
-                const int c = j0 + wn * S::WTN + fn * 16 + (lane & 15);
-                store_frag<AR>(C, p.ldc, r0, c, acc[fm][fn], p, c_fast, p.m, p.n);
-            }
+        const TileJob j = make_job(p, inst, (rem % p.tiles_m) * BM, (rem / p.tiles_m) * BN);
+        tile_job<AR, AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N>(j, smem);
     }
 }
 
 // ---------------------------------------------------------------------------
-// Direct (LDS-free) kernel: one wave per WTM x WTN block.
+// Direct (LDS-free) tiles: one wave per WTM x WTN block.
 // ---------------------------------------------------------------------------
 
 // Load the lane's operand fragment for one MFMA k-step straight from global:
@@ -444,57 +478,61 @@ __device__ __forceinline__ void direct_frag(const char* base, long sr, long sk, 
     }
 }
 
+// One wave computes the WTM x WTN block of C at (j.i0, j.j0).
 template <class AR, bool AKC, bool BKC, int WTM, int WTN>
-__global__ void __launch_bounds__(256)
-direct_gemm(const KParams p)
+__device__ __forceinline__ void direct_job(const TileJob& j)
 {
     using Out = typename AR::Out;
     constexpr int FM = WTM / 16, FN = WTN / 16;
     const int lane = threadIdx.x & 63;
-    const long sra = AKC ? p.lda : 1, ska = AKC ? 1 : p.lda;
-    const long srb = BKC ? p.ldb : 1, skb = BKC ? 1 : p.ldb;
+    const long sra = AKC ? j.lda : 1, ska = AKC ? 1 : j.lda;
+    const long srb = BKC ? j.ldb : 1, skb = BKC ? 1 : j.ldb;
+    const bool a_vec = ((j.lda * AR::E) % 16) == 0 && aligned16(j.A);
+    const bool b_vec = ((j.ldb * AR::E) % 16) == 0 && aligned16(j.B);
+
+    typename AR::Acc acc[FM][FN];
+#pragma unroll
+    for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = typename AR::Acc{};
+
+    for (int kb = 0; kb < j.k; kb += AR::KSTEP) {
+        u32x4 af[FM][AR::NPLANE], bfg[FN][AR::NPLANE];
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm)
+            direct_frag<AR, AKC>(j.A, sra, ska, j.i0 + fm * 16 + frag_row<AR>(lane & 15), j.m, kb, j.k, a_vec,
+                                 af[fm]);
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+            direct_frag<AR, BKC>(j.B, srb, skb, j.j0 + fn * 16 + (lane & 15), j.n, kb, j.k, b_vec, bfg[fn]);
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
+    }
+
+    Out* C = static_cast<Out*>(j.C);
+    const bool c_fast = j.i0 + WTM <= j.m && j.j0 + WTN <= j.n && ((j.ldc * (long)sizeof(Out)) % 16) == 0 &&
+                        aligned16(C);
+#pragma unroll
+    for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+            store_frag<AR>(C, j.ldc, j.i0 + fm * 16 + 4 * (lane >> 4), j.j0 + fn * 16 + (lane & 15), acc[fm][fn],
+                           j.s, j.beta0 != 0, c_fast, j.m, j.n);
+}
+
+template <class AR, bool AKC, bool BKC, int WTM, int WTN>
+__global__ void __launch_bounds__(256)
+direct_gemm(const KParams p)
+{
     const int per_inst = p.tiles_m * p.tiles_n;
     const long nwaves = (long)gridDim.x * 4;
-
     for (long t = (long)blockIdx.x * 4 + (threadIdx.x >> 6); t < p.total; t += nwaves) {
         const int inst = (int)(t / per_inst);
         const int rem = (int)(t % per_inst);
-        const int i0 = (rem % p.tiles_m) * WTM;
-        const int j0 = (rem / p.tiles_m) * WTN;
-        const char* A = static_cast<const char*>(p.A[inst]);
-        const char* B = static_cast<const char*>(p.B[inst]);
-        Out* C = static_cast<Out*>(p.C[inst]);
-        const bool a_vec = p.ld_vec_a && aligned16(A);
-        const bool b_vec = p.ld_vec_b && aligned16(B);
-
-        typename AR::Acc acc[FM][FN];
-#pragma unroll
-        for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = typename AR::Acc{};
-
-        for (int kb = 0; kb < p.k; kb += AR::KSTEP) {
-            u32x4 af[FM][AR::NPLANE], bfg[FN][AR::NPLANE];
-#pragma unroll
-            for (int fm = 0; fm < FM; ++fm)
-                direct_frag<AR, AKC>(A, sra, ska, i0 + fm * 16 + frag_row<AR>(lane & 15), p.m, kb, p.k, a_vec,
-                                     af[fm]);
-#pragma unroll
-            for (int fn = 0; fn < FN; ++fn)
-                direct_frag<AR, BKC>(B, srb, skb, j0 + fn * 16 + (lane & 15), p.n, kb, p.k, b_vec, bfg[fn]);
-#pragma unroll
-            for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-                for (int fn = 0; fn < FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
-        }
-
-        const bool c_fast = i0 + WTM <= p.m && j0 + WTN <= p.n && p.ld_vec_c && aligned16(C);
-#pragma unroll
-        for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < FN; ++fn)
-                store_frag<AR>(C, p.ldc, i0 + fm * 16 + 4 * (lane >> 4), j0 + fn * 16 + (lane & 15),
-                               acc[fm][fn], p, c_fast, p.m, p.n);
+        const TileJob j = make_job(p, inst, (rem % p.tiles_m) * WTM, (rem / p.tiles_m) * WTN);
+        direct_job<AR, AKC, BKC, WTM, WTN>(j);
     }
 }
 

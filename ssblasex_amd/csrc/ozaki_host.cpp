@@ -1,10 +1,60 @@
-// ozaki_host.cpp -- placeholder until the Ozaki kernels land.
+// ozaki_host.cpp -- workspace management for the Ozaki fp64 form.
+//
+// The slices and exponents live in a device buffer owned by the library, one
+// per (device, stream) so that calls on different streams never share it.
+// A buffer only grows; growing waits for the device to go idle before the
+// old buffer is released.  Calls on one stream are ordered by the stream.
+#include <map>
+#include <mutex>
+#include <utility>
+
 #include "api_internal.hpp"
 
 namespace ssb {
-hipError_t launch_ozaki(const GemmProblem&, int, hipStream_t, ssblasStatus_t* st)
+
+namespace {
+struct Buf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+std::mutex g_mu;
+std::map<std::pair<int, hipStream_t>, Buf> g_bufs;
+thread_local hipStream_t t_stream = nullptr;
+
+void* get_ws(size_t bytes, hipError_t* err)
 {
-    *st = SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
-    return hipSuccess;
+    int dev = 0;
+    *err = hipGetDevice(&dev);
+    if (*err != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(g_mu);
+    Buf& b = g_bufs[{dev, t_stream}];
+    if (b.bytes >= bytes) return b.ptr;
+    if (b.ptr != nullptr) {
+        *err = hipDeviceSynchronize();
+        if (*err != hipSuccess) return nullptr;
+        (void)hipFree(b.ptr);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    size_t want = bytes + bytes / 8;  // headroom for slightly larger calls
+    *err = hipMalloc(&b.ptr, want);
+    if (*err != hipSuccess) {
+        b.ptr = nullptr;
+        return nullptr;
+    }
+    b.bytes = want;
+    return b.ptr;
 }
+}  // namespace
+
+hipError_t launch_ozaki(const GemmProblem& g, int slices, hipStream_t stream, ssblasStatus_t* st)
+{
+    *st = SSBLAS_STATUS_SUCCESS;
+    const int s = slices > 0 ? slices : OZAKI_DEFAULT_SLICES;
+    t_stream = stream;
+    hipError_t e = ozaki_run(g, s, get_ws, stream);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) *st = SSBLAS_STATUS_ALLOC_FAILED;
+    return e;
+}
+
 }  // namespace ssb

@@ -1,0 +1,97 @@
+"""GPU checks of the Ozaki fp64 form (SSBLAS_COMPUTE_64F_EMULATED_OZAKI*).
+
+The form has no counterpart in the reference; its parity bar is the
+north-star one: normwise error against the FP64 (double-double) oracle,
+reported per slice count, and at the default slice count (8) no worse than
+2 x the reference-order fp64 restatement's error + 1 unit roundoff.  In
+addition the library must reproduce tests/ozaki_model.py (the same
+algorithm in numpy) bit for bit when alpha is a power of two.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import UNIT_ROUNDOFF, make_batch, normwise, padding_untouched, run_gpu, run_oracle
+from ozaki_model import ozaki_gemm
+
+pytestmark = pytest.mark.gpu
+OPS = [(0, 0), (0, 1), (1, 0), (1, 1)]
+
+
+def op_mats(b, i):
+    A = oracle.colmajor(b.A[i], *b.dims_a(), b.lda).astype(np.float64)
+    B = oracle.colmajor(b.B[i], *b.dims_b(), b.ldb).astype(np.float64)
+    return (A if b.transa == 0 else A.T), (B if b.transb == 0 else B.T)
+
+
+@pytest.mark.parametrize("ta,tb", OPS)
+@pytest.mark.parametrize("s", [3, 5, 8])
+@pytest.mark.parametrize("shape", [(16, 16, 32), (37, 29, 70), (64, 64, 64), (100, 65, 129)], ids=str)
+def test_ozaki_bit_exact_vs_model(ta, tb, s, shape):
+    m, n, k = shape
+    b = make_batch("f64", ta, tb, m, n, k, 2, alpha=2.0, beta=-0.5, pad=(1, 2, 3), seed=s * 7 + m, dist="wide")
+    st, C = run_gpu(b, compute_type=0x100 + s)
+    assert st == 0
+    assert padding_untouched(b, C)
+    for i in range(b.batch):
+        A, B = op_mats(b, i)
+        C0 = oracle.colmajor(b.C0[i], m, n, b.ldc)
+        want = ozaki_gemm(A, B, s, alpha=2.0, beta=-0.5, C0=C0)
+        got = oracle.colmajor(C[i], m, n, b.ldc)
+        assert np.array_equal(got, want), (i, np.max(np.abs(got - want)))
+
+
+@pytest.mark.parametrize("ta,tb", OPS)
+def test_ozaki_error_decreases_and_meets_native(ta, tb):
+    b = make_batch("f64", ta, tb, 48, 40, 96, 3, alpha=1.0, beta=0.0, seed=3, dist="wide")
+    errs = {}
+    for s in range(3, 10):
+        st, C = run_gpu(b, compute_type=0x100 + s)
+        assert st == 0
+        errs[s] = normwise(b, C)
+    for s in range(3, 8):
+        assert errs[s + 1] < errs[s] or errs[s + 1] <= UNIT_ROUNDOFF["f64"], errs
+    _, R = run_oracle(b)
+    e_ref = normwise(b, R)
+    st, C = run_gpu(b, compute_type=6)  # default slice count (9)
+    assert st == 0
+    assert normwise(b, C) <= 2 * e_ref + UNIT_ROUNDOFF["f64"], (normwise(b, C), e_ref, errs)
+
+
+def test_ozaki_nonfinite_rows_become_nan():
+    b = make_batch("f64", 0, 0, 20, 12, 30, 1, alpha=1.0, beta=0.0, seed=4)
+    oracle.colmajor(b.A[0], 20, 30, b.lda)[3, 7] = np.inf
+    st, C = run_gpu(b, compute_type=6)
+    assert st == 0
+    out = oracle.colmajor(C[0], 20, 12, b.ldc)
+    assert np.isnan(out[3]).all()
+    assert np.isfinite(np.delete(out, 3, axis=0)).all()
+
+
+def test_ozaki_long_k_chunked():
+    """K beyond one int32-exact chunk (8192) is folded chunk by chunk."""
+    b = make_batch("f64", 1, 0, 8, 8, 9000, 1, alpha=1.0, beta=0.0, seed=5)
+    st, C = run_gpu(b, compute_type=0x108)
+    assert st == 0
+    _, R = run_oracle(b)
+    assert normwise(b, C) <= 2 * normwise(b, R) + UNIT_ROUNDOFF["f64"]
+
+
+def test_ozaki_config_fp64_32cubed_sample():
+    """fp64 NT m=n=k=32, entries U[-1,1] * 2^U{-8..8}: error per slice count
+    on 256 instances (the benchmark configuration's shape)."""
+    b = make_batch("f64", 0, 1, 32, 32, 32, 256, alpha=1.0, beta=0.0, seed=0x55B1A5 + 3, dist="wide")
+    st, C = run_gpu(b)
+    e_native = normwise(b, C)
+    _, R = run_oracle(b, 8)
+    e_ref = normwise(b, R)
+    assert e_native <= 2 * e_ref + UNIT_ROUNDOFF["f64"]
+    res = {}
+    for s in range(4, 9):
+        st, C = run_gpu(b, compute_type=0x100 + s)
+        assert st == 0
+        res[s] = normwise(b, C)
+    print("ozaki normwise error by slice count", res, "native", e_native, "reference order", e_ref)
+    assert res[8] <= 2 * e_ref + UNIT_ROUNDOFF["f64"]
