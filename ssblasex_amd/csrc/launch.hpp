@@ -16,15 +16,18 @@ enum ArithKind : int {
 };
 
 // Tile configurations.  DIRECT_*: one wave per block, no LDS.  TILE_*: LDS
-// double-buffered workgroup tiles.  Per-type k-depths are fixed per config.
+// double-buffered workgroup tiles (k-depth per form in launch_impl.hpp).
 enum TileConfig : int {
-    CFG_DIRECT_16 = 0,   // wave per 16x16
-    CFG_DIRECT_32 = 1,   // wave per 32x32
-    CFG_TILE_64 = 2,     // 64x64 tile, 4 waves of 32x32
-    CFG_TILE_128 = 3,    // 128x128 tile, 4 waves of 64x64
-    CFG_TILE_128x64 = 4, // 128x64 tile (M x N), 4 waves of 64x32
-    CFG_TILE_64x128 = 5, // 64x128 tile, 4 waves of 32x64
-    CFG_COUNT = 6
+    CFG_DIRECT_16 = 0,     // wave per 16x16
+    CFG_DIRECT_32 = 1,     // wave per 32x32
+    CFG_TILE_64 = 2,       // 64x64, 4 waves of 32x32
+    CFG_TILE_128 = 3,      // 128x128, 4 waves of 64x64
+    CFG_TILE_128x64 = 4,   // 128x64 (M x N), 4 waves of 64x32
+    CFG_TILE_64x128 = 5,   // 64x128, 4 waves of 32x64
+    CFG_TILE_128_K2 = 6,   // 128x128, 4 waves, twice the k-depth of TILE_128
+    CFG_TILE_128_W8 = 7,   // 128x128, 8 waves of 32x64
+    CFG_TILE_256x128 = 8,  // 256x128, 8 waves of 64x64
+    CFG_COUNT = 9
 };
 
 const char* config_name(int cfg);

@@ -7,14 +7,40 @@
 
 namespace ssb {
 
-// k-depth per (form, config) for the LDS-tiled configs.
+// k-depth per (form, config) for the LDS-tiled configs: base depth per form
+// (int8 64, fp32 16, fp64 16, bf16 32), doubled where the config says so,
+// and halved again while the double-buffered tile would not fit the 160 KiB
+// of LDS (never below one MFMA k-step).
+template <class AR>
+constexpr int base_k()
+{
+    return AR::ID == ARITH_I8 ? 64 : AR::ID == ARITH_F64 ? 16 : AR::ID == ARITH_F32 ? 16 : 32;
+}
+template <class AR, int BM, int BN, int WM, int WN>
+constexpr int fit_k(int bk)
+{
+    while (bk > AR::KSTEP &&
+           2 * AR::NPLANE * (BM + BN) * (bk * AR::LDS_E + 16) > 160 * 1024)
+        bk /= 2;
+    return bk;
+}
 template <class AR, int CFG>
-struct TileK {
-    static constexpr int value =
-        AR::ID == ARITH_I8 ? 64 :
-        AR::ID == ARITH_F64 ? 16 :
-        AR::ID == ARITH_F32 ? (CFG == CFG_TILE_64 ? 32 : 16) : 32;
-};
+struct CfgShape;
+#define SSB_CFG(C, BM_, BN_, WM_, WN_, KMUL)                                         \
+    template <class AR>                                                               \
+    struct CfgShape<AR, C> {                                                          \
+        static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_;                  \
+        static constexpr int BK = fit_k<AR, BM_, BN_, WM_, WN_>(base_k<AR>() * KMUL); \
+        static constexpr bool FITS = 2 * AR::NPLANE * (BM_ + BN_) * (BK * AR::LDS_E + 16) <= 160 * 1024; \
+    };
+SSB_CFG(CFG_TILE_64, 64, 64, 2, 2, (AR::ID == ARITH_F32 ? 2 : 1))
+SSB_CFG(CFG_TILE_128, 128, 128, 2, 2, 1)
+SSB_CFG(CFG_TILE_128x64, 128, 64, 2, 2, (AR::ID == ARITH_F32 ? 2 : 1))
+SSB_CFG(CFG_TILE_64x128, 64, 128, 2, 2, (AR::ID == ARITH_F32 ? 2 : 1))
+SSB_CFG(CFG_TILE_128_K2, 128, 128, 2, 2, 2)
+SSB_CFG(CFG_TILE_128_W8, 128, 128, 4, 2, (AR::ID == ARITH_F32 ? 2 : 1))
+SSB_CFG(CFG_TILE_256x128, 256, 128, 4, 2, 1)
+#undef SSB_CFG
 
 template <class K>
 struct OccCache {
@@ -43,17 +69,23 @@ inline long persistent_grid(long work, int per_cu)
     return slots >= 8 ? (slots / 8) * 8 : slots;
 }
 
-template <class AR, bool AKC, bool BKC, int BM, int BN, int BK>
+template <class AR, bool AKC, bool BKC, int CFG>
 hipError_t run_tile(KParams p, hipStream_t st)
 {
-    auto kern = tile_gemm<AR, AKC, BKC, BM, BN, BK, 2, 2>;
-    p.tiles_m = (p.m + BM - 1) / BM;
-    p.tiles_n = (p.n + BN - 1) / BN;
+    using CS = CfgShape<AR, CFG>;
+    if constexpr (!CS::FITS) {
+        return run_tile<AR, AKC, BKC, CFG_TILE_128>(p, st);  // tile does not fit in LDS for this form
+    } else {
+    constexpr int NT = 64 * CS::WM * CS::WN;
+    auto kern = tile_gemm<AR, AKC, BKC, CS::BM, CS::BN, CS::BK, CS::WM, CS::WN>;
+    p.tiles_m = (p.m + CS::BM - 1) / CS::BM;
+    p.tiles_n = (p.n + CS::BN - 1) / CS::BN;
     p.total = (long)p.batch * p.tiles_m * p.tiles_n;
-    int occ = OccCache<decltype(kern)>::get(kern, 256);
+    int occ = OccCache<decltype(kern)>::get(kern, NT);
     long grid = persistent_grid(p.total, occ);
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, p);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), 0, st, p);
     return hipGetLastError();
+    }
 }
 
 template <class AR, bool AKC, bool BKC, int W>
@@ -76,10 +108,13 @@ hipError_t run_cfg(const KParams& p, int cfg, hipStream_t st)
     switch (cfg) {
     case CFG_DIRECT_16: return run_direct<AR, AKC, BKC, 16>(p, st);
     case CFG_DIRECT_32: return run_direct<AR, AKC, BKC, 32>(p, st);
-    case CFG_TILE_64: return run_tile<AR, AKC, BKC, 64, 64, TileK<AR, CFG_TILE_64>::value>(p, st);
-    case CFG_TILE_128: return run_tile<AR, AKC, BKC, 128, 128, TileK<AR, CFG_TILE_128>::value>(p, st);
-    case CFG_TILE_128x64: return run_tile<AR, AKC, BKC, 128, 64, TileK<AR, CFG_TILE_128x64>::value>(p, st);
-    case CFG_TILE_64x128: return run_tile<AR, AKC, BKC, 64, 128, TileK<AR, CFG_TILE_64x128>::value>(p, st);
+    case CFG_TILE_64: return run_tile<AR, AKC, BKC, CFG_TILE_64>(p, st);
+    case CFG_TILE_128: return run_tile<AR, AKC, BKC, CFG_TILE_128>(p, st);
+    case CFG_TILE_128x64: return run_tile<AR, AKC, BKC, CFG_TILE_128x64>(p, st);
+    case CFG_TILE_64x128: return run_tile<AR, AKC, BKC, CFG_TILE_64x128>(p, st);
+    case CFG_TILE_128_K2: return run_tile<AR, AKC, BKC, CFG_TILE_128_K2>(p, st);
+    case CFG_TILE_128_W8: return run_tile<AR, AKC, BKC, CFG_TILE_128_W8>(p, st);
+    case CFG_TILE_256x128: return run_tile<AR, AKC, BKC, CFG_TILE_256x128>(p, st);
     default: return hipErrorInvalidValue;
     }
 }
