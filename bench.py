@@ -183,7 +183,12 @@ def secondary(args, stream):
         ("fp32 NN m=n=k=64 batch 128 beta=0.5", dict(t="f32", ta=0, tb=0, m=64, n=64, k=64, batch=128, alpha=1.0,
                                                      beta=0.5, seed=SEED + 1), [("native", 0, "f32")]),
         ("fp64 NT m=n=k=32 batch 65536", dict(t="f64", ta=0, tb=1, m=32, n=32, k=32, batch=65536, alpha=1.0,
-                                              beta=0.0, seed=SEED + 3, dist="wide"), [("native", 0, "f64")]),
+                                              beta=0.0, seed=SEED + 3, dist="wide"),
+         [("native", 0, "f64"), ("ozaki_s6", 0x106, "f64"), ("ozaki_s7", 0x107, "f64"), ("ozaki_s8", 0x108, "f64"),
+          ("ozaki_s9", 0x109, "f64")]),
+        ("fp64 NN m=n=k=256 batch 512", dict(t="f64", ta=0, tb=0, m=256, n=256, k=256, batch=512, alpha=1.0,
+                                             beta=0.0, seed=SEED + 5, dist="wide"),
+         [("native", 0, "f64"), ("ozaki_s7", 0x107, "f64"), ("ozaki_s8", 0x108, "f64"), ("ozaki_s9", 0x109, "f64")]),
         ("fp32 TN m=n=k=256 batch 4096 alpha=1.5 beta=-0.5",
          dict(t="f32", ta=1, tb=0, m=256, n=256, k=256, batch=4096, alpha=1.5, beta=-0.5, seed=SEED + 4,
               dist="normal"), [("native", 0, "f32"), ("bf16x3", 4, "bf16"), ("bf16x6", 5, "bf16")]),

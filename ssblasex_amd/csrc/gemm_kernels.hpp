@@ -294,7 +294,12 @@ struct TileShape {
 // LDS-staged workgroup tile: BM x BN of C, all threads of the workgroup.
 // smem must hold TileShape::LDS_BYTES; the caller's previous use of it must
 // be finished by every thread (the routine starts with a barrier).
-template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N>
+//
+// STAGES = 1: the global loads of k-block kt+1 are in flight while block kt
+// is multiplied.  STAGES = 2: two register staging sets, loads issued two
+// k-blocks ahead (covers twice the memory latency for twice the staging
+// registers).
+template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES = 1>
 __device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
 {
     using S = TileShape<AR, AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N>;
@@ -324,38 +329,30 @@ __device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
     const char* Ab = j.A + (long)j.i0 * sra * E;
     const char* Bb = j.B + (long)j.j0 * srb * E;
 
-    SA sa;
-    SB sb;
+    SA sa[STAGES];
+    SB sb[STAGES];
     typename AR::Acc acc[S::FM][S::FN];
 #pragma unroll
     for (int fm = 0; fm < S::FM; ++fm)
 #pragma unroll
         for (int fn = 0; fn < S::FN; ++fn) acc[fm][fn] = typename AR::Acc{};
 
-    auto load_step = [&](int kt) {
+    auto load_step = [&](int kt, SA& xa, SB& xb) {
         const int k0 = kt * BK;
         const int kv = j.k - k0;
         const char* ak = Ab + (long)k0 * ska * E;
         const char* bk = Bb + (long)k0 * skb * E;
-        if (a_fast && kv >= BK) sa.load_fast(ak, sra, ska, tid);
-        else sa.load_guarded(ak, sra, ska, rows_a, kv, a_vec, tid);
-        if (b_fast && kv >= BK) sb.load_fast(bk, srb, skb, tid);
-        else sb.load_guarded(bk, srb, skb, rows_b, kv, b_vec, tid);
+        if (a_fast && kv >= BK) xa.load_fast(ak, sra, ska, tid);
+        else xa.load_guarded(ak, sra, ska, rows_a, kv, a_vec, tid);
+        if (b_fast && kv >= BK) xb.load_fast(bk, srb, skb, tid);
+        else xb.load_guarded(bk, srb, skb, rows_b, kv, b_vec, tid);
     };
-    auto store_step = [&](int buf) {
+    auto store_step = [&](int buf, const SA& xa, const SB& xb) {
         char* base = smem + buf * S::BUF;
-        sa.store(base, S::A_PLANE, S::ROWB, tid);
-        sb.store(base + AR::NPLANE * S::A_PLANE, S::B_PLANE, S::ROWB, tid);
+        xa.store(base, S::A_PLANE, S::ROWB, tid);
+        xb.store(base + AR::NPLANE * S::A_PLANE, S::B_PLANE, S::ROWB, tid);
     };
-
-    load_step(0);
-    __syncthreads();  // previous user of smem is done
-    store_step(0);
-    __syncthreads();
-
-    for (int kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) load_step(kt + 1);
+    auto compute = [&](int buf) {
         const char* la = smem + buf * S::BUF;
         const char* lb = la + AR::NPLANE * S::A_PLANE;
 #pragma unroll
@@ -381,8 +378,40 @@ __device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
 #pragma unroll
                 for (int fn = 0; fn < S::FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
         }
-        if (kt + 1 < nk) store_step(buf ^ 1);
+    };
+
+    if constexpr (STAGES == 1) {
+        load_step(0, sa[0], sb[0]);
+        __syncthreads();  // previous user of smem is done
+        store_step(0, sa[0], sb[0]);
         __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            const int buf = kt & 1;
+            if (kt + 1 < nk) load_step(kt + 1, sa[0], sb[0]);
+            compute(buf);
+            if (kt + 1 < nk) store_step(buf ^ 1, sa[0], sb[0]);
+            __syncthreads();
+        }
+    } else {
+        // staging set of k-block u is u % 2; LDS buffer of k-block u is u % 2
+        load_step(0, sa[0], sb[0]);
+        if (nk > 1) load_step(1, sa[1], sb[1]);
+        __syncthreads();
+        store_step(0, sa[0], sb[0]);
+        __syncthreads();
+        for (int kt = 0; kt < nk; kt += 2) {
+            // even k-block kt: set 0 is in LDS buffer 0, set 1 holds kt+1
+            if (kt + 2 < nk) load_step(kt + 2, sa[0], sb[0]);
+            compute(0);
+            if (kt + 1 < nk) store_step(1, sa[1], sb[1]);
+            __syncthreads();
+            if (kt + 1 >= nk) break;
+            // odd k-block kt+1: set 1 is in LDS buffer 1, set 0 holds kt+2
+            if (kt + 3 < nk) load_step(kt + 3, sa[1], sb[1]);
+            compute(1);
+            if (kt + 2 < nk) store_step(0, sa[0], sb[0]);
+            __syncthreads();
+        }
     }
 
     // fused alpha/beta epilogue
@@ -418,7 +447,7 @@ __device__ __forceinline__ TileJob make_job(const KParams& p, int inst, int i0, 
 }
 
 // Batched persistent kernel: workgroups walk the (instance, tile) work ids.
-template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N>
+template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES = 1>
 __global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
 tile_gemm(const KParams p)
 {
@@ -430,7 +459,7 @@ tile_gemm(const KParams p)
         const int inst = (int)(w / tiles_per_inst);
         const int rem = (int)(w % tiles_per_inst);
         const TileJob j = make_job(p, inst, (rem % p.tiles_m) * BM, (rem / p.tiles_m) * BN);
-        tile_job<AR, AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N>(j, smem);
+        tile_job<AR, AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, STAGES>(j, smem);
     }
 }
 

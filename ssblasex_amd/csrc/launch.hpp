@@ -27,7 +27,13 @@ enum TileConfig : int {
     CFG_TILE_128_K2 = 6,   // 128x128, 4 waves, twice the k-depth of TILE_128
     CFG_TILE_128_W8 = 7,   // 128x128, 8 waves of 32x64
     CFG_TILE_256x128 = 8,  // 256x128, 8 waves of 64x64
-    CFG_COUNT = 9
+    // the same shapes with loads issued two k-blocks ahead
+    CFG_TILE_64_S2 = 9,
+    CFG_TILE_128_S2 = 10,
+    CFG_TILE_128_K2_S2 = 11,
+    CFG_TILE_128_W8_S2 = 12,
+    CFG_TILE_256x128_S2 = 13,
+    CFG_COUNT = 14
 };
 
 const char* config_name(int cfg);

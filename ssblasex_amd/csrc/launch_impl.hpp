@@ -26,20 +26,25 @@ constexpr int fit_k(int bk)
 }
 template <class AR, int CFG>
 struct CfgShape;
-#define SSB_CFG(C, BM_, BN_, WM_, WN_, KMUL)                                         \
+#define SSB_CFG(C, BM_, BN_, WM_, WN_, KMUL, ST)                                     \
     template <class AR>                                                               \
     struct CfgShape<AR, C> {                                                          \
-        static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_;                  \
+        static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_, STAGES = ST;     \
         static constexpr int BK = fit_k<AR, BM_, BN_, WM_, WN_>(base_k<AR>() * KMUL); \
         static constexpr bool FITS = 2 * AR::NPLANE * (BM_ + BN_) * (BK * AR::LDS_E + 16) <= 160 * 1024; \
     };
-SSB_CFG(CFG_TILE_64, 64, 64, 2, 2, (AR::ID == ARITH_F32 ? 2 : 1))
-SSB_CFG(CFG_TILE_128, 128, 128, 2, 2, 1)
-SSB_CFG(CFG_TILE_128x64, 128, 64, 2, 2, (AR::ID == ARITH_F32 ? 2 : 1))
-SSB_CFG(CFG_TILE_64x128, 64, 128, 2, 2, (AR::ID == ARITH_F32 ? 2 : 1))
-SSB_CFG(CFG_TILE_128_K2, 128, 128, 2, 2, 2)
-SSB_CFG(CFG_TILE_128_W8, 128, 128, 4, 2, (AR::ID == ARITH_F32 ? 2 : 1))
-SSB_CFG(CFG_TILE_256x128, 256, 128, 4, 2, 1)
+SSB_CFG(CFG_TILE_64, 64, 64, 2, 2, (AR::ID == ARITH_F32 ? 2 : 1), 1)
+SSB_CFG(CFG_TILE_128, 128, 128, 2, 2, 1, 1)
+SSB_CFG(CFG_TILE_128x64, 128, 64, 2, 2, (AR::ID == ARITH_F32 ? 2 : 1), 1)
+SSB_CFG(CFG_TILE_64x128, 64, 128, 2, 2, (AR::ID == ARITH_F32 ? 2 : 1), 1)
+SSB_CFG(CFG_TILE_128_K2, 128, 128, 2, 2, 2, 1)
+SSB_CFG(CFG_TILE_128_W8, 128, 128, 4, 2, (AR::ID == ARITH_F32 ? 2 : 1), 1)
+SSB_CFG(CFG_TILE_256x128, 256, 128, 4, 2, 1, 1)
+SSB_CFG(CFG_TILE_64_S2, 64, 64, 2, 2, (AR::ID == ARITH_F32 ? 2 : 1), 2)
+SSB_CFG(CFG_TILE_128_S2, 128, 128, 2, 2, 1, 2)
+SSB_CFG(CFG_TILE_128_K2_S2, 128, 128, 2, 2, 2, 2)
+SSB_CFG(CFG_TILE_128_W8_S2, 128, 128, 4, 2, (AR::ID == ARITH_F32 ? 2 : 1), 2)
+SSB_CFG(CFG_TILE_256x128_S2, 256, 128, 4, 2, 1, 2)
 #undef SSB_CFG
 
 template <class K>
@@ -77,7 +82,7 @@ hipError_t run_tile(KParams p, hipStream_t st)
         return run_tile<AR, AKC, BKC, CFG_TILE_128>(p, st);  // tile does not fit in LDS for this form
     } else {
     constexpr int NT = 64 * CS::WM * CS::WN;
-    auto kern = tile_gemm<AR, AKC, BKC, CS::BM, CS::BN, CS::BK, CS::WM, CS::WN>;
+    auto kern = tile_gemm<AR, AKC, BKC, CS::BM, CS::BN, CS::BK, CS::WM, CS::WN, CS::STAGES>;
     p.tiles_m = (p.m + CS::BM - 1) / CS::BM;
     p.tiles_n = (p.n + CS::BN - 1) / CS::BN;
     p.total = (long)p.batch * p.tiles_m * p.tiles_n;
@@ -115,6 +120,11 @@ hipError_t run_cfg(const KParams& p, int cfg, hipStream_t st)
     case CFG_TILE_128_K2: return run_tile<AR, AKC, BKC, CFG_TILE_128_K2>(p, st);
     case CFG_TILE_128_W8: return run_tile<AR, AKC, BKC, CFG_TILE_128_W8>(p, st);
     case CFG_TILE_256x128: return run_tile<AR, AKC, BKC, CFG_TILE_256x128>(p, st);
+    case CFG_TILE_64_S2: return run_tile<AR, AKC, BKC, CFG_TILE_64_S2>(p, st);
+    case CFG_TILE_128_S2: return run_tile<AR, AKC, BKC, CFG_TILE_128_S2>(p, st);
+    case CFG_TILE_128_K2_S2: return run_tile<AR, AKC, BKC, CFG_TILE_128_K2_S2>(p, st);
+    case CFG_TILE_128_W8_S2: return run_tile<AR, AKC, BKC, CFG_TILE_128_W8_S2>(p, st);
+    case CFG_TILE_256x128_S2: return run_tile<AR, AKC, BKC, CFG_TILE_256x128_S2>(p, st);
     default: return hipErrorInvalidValue;
     }
 }
