@@ -26,6 +26,11 @@ const char* config_name(int cfg)
     case CFG_TILE_128_K2_S2: return "tile128k2s2";
     case CFG_TILE_128_W8_S2: return "tile128w8s2";
     case CFG_TILE_256x128_S2: return "tile256x128s2";
+    case CFG_DMA_64: return "dma64";
+    case CFG_DMA_128: return "dma128";
+    case CFG_DMA_128_K2: return "dma128k2";
+    case CFG_DMA_256x128: return "dma256x128";
+    case CFG_DMA_128_S3: return "dma128s3";
     default: return "unknown";
     }
 }
@@ -34,7 +39,8 @@ void config_tile(int cfg, int* tm, int* tn)
 {
     static const int dims[CFG_COUNT][2] = {{16, 16},   {32, 32},   {64, 64},   {128, 128}, {128, 64},
                                            {64, 128},  {128, 128}, {128, 128}, {256, 128}, {64, 64},
-                                           {128, 128}, {128, 128}, {128, 128}, {256, 128}};
+                                           {128, 128}, {128, 128}, {128, 128}, {256, 128}, {64, 64},
+                                           {128, 128}, {128, 128}, {256, 128}, {128, 128}};
     if (cfg < 0 || cfg >= CFG_COUNT) cfg = CFG_TILE_64;
     *tm = dims[cfg][0];
     *tn = dims[cfg][1];

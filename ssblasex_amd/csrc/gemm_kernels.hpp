@@ -64,7 +64,7 @@ __device__ __forceinline__ typename Raw<E>::T load_elem(const char* base, long o
 //                          registers before the LDS store.
 // ---------------------------------------------------------------------------
 template <class AR, bool KC, int ROWS, int BK, int NT>
-struct Stager {
+struct StagerImpl {
     static constexpr int E = AR::E;
     static constexpr int WB = (!KC && E == 1) ? 4 : 16;  // bytes per word
     static constexpr int WE = WB / E;                     // elements per word
@@ -76,10 +76,10 @@ struct Stager {
     static constexpr int KG = BK / KU;
     static constexpr int UNITS = RG * KG;
     static constexpr int UPT = (UNITS + NT - 1) / NT;
+    static constexpr int UD = NW * WD;                    // dwords per unit
+    static constexpr int RAWD = UPT * UD;                 // dwords per thread
     static_assert(BK % KU == 0, "BK must hold whole units");
     static_assert(ROWS % RU == 0, "rows must hold whole units");
-
-    uint32_t raw[UPT][NW * WD];
 
     __device__ __forceinline__ static void unit_coords(int u, int& r0, int& k0)
     {
@@ -95,19 +95,21 @@ struct Stager {
         else return ((long)r0 + (long)(k0 + w) * sk) * E;
     }
 
-    __device__ __forceinline__ void put_word(int x, int w, const char* src)
+    __device__ __forceinline__ static void put_word(uint32_t* raw, int x, int w, const char* src)
     {
         if constexpr (WB == 16) {
             u32x4 v = *reinterpret_cast<const u32x4*>(src);
-#pragma unroll
-            for (int d = 0; d < 4; ++d) raw[x][w * 4 + d] = v[d];
+            raw[x * UD + w * 4 + 0] = v.x;
+            raw[x * UD + w * 4 + 1] = v.y;
+            raw[x * UD + w * 4 + 2] = v.z;
+            raw[x * UD + w * 4 + 3] = v.w;
         } else {
-            raw[x][w] = *reinterpret_cast<const uint32_t*>(src);
+            raw[x * UD + w] = *reinterpret_cast<const uint32_t*>(src);
         }
     }
 
     // Every row and k of this tile step in range, words aligned.
-    __device__ __forceinline__ void load_fast(const char* base, long sr, long sk, int tid)
+    __device__ __forceinline__ static void load_fast(uint32_t* raw, const char* base, long sr, long sk, int tid)
     {
 #pragma unroll
         for (int x = 0; x < UPT; ++x) {
@@ -116,14 +118,14 @@ struct Stager {
             int r0, k0;
             unit_coords(u, r0, k0);
 #pragma unroll
-            for (int w = 0; w < NW; ++w) put_word(x, w, base + word_off(r0, k0, w, sr, sk));
+            for (int w = 0; w < NW; ++w) put_word(raw, x, w, base + word_off(r0, k0, w, sr, sk));
         }
     }
 
     // Zero outside rows_valid x k_valid; a word is read with one vector load
     // when it is whole and `vec`, else element by element.
-    __device__ __forceinline__ void load_guarded(const char* base, long sr, long sk, int rows_valid,
-                                                 int k_valid, bool vec, int tid)
+    __device__ __forceinline__ static void load_guarded(uint32_t* raw, const char* base, long sr, long sk,
+                                                        int rows_valid, int k_valid, bool vec, int tid)
     {
         using RT = typename Raw<E>::T;
 #pragma unroll
@@ -138,7 +140,7 @@ struct Stager {
                 if constexpr (KC) whole = r0 < rows_valid && k0 + WE <= k_valid;
                 else whole = r0 + RU <= rows_valid && k0 + w < k_valid;
                 if (vec && whole) {
-                    put_word(x, w, base + word_off(r0, k0, w, sr, sk));
+                    put_word(raw, x, w, base + word_off(r0, k0, w, sr, sk));
                 } else {
                     RT vals[WE];
 #pragma unroll
@@ -157,14 +159,14 @@ struct Stager {
                     uint32_t d[WD];
                     __builtin_memcpy(d, vals, WB);
 #pragma unroll
-                    for (int i = 0; i < WD; ++i) raw[x][w * WD + i] = d[i];
+                    for (int i = 0; i < WD; ++i) raw[x * UD + w * WD + i] = d[i];
                 }
             }
         }
     }
 
     // Write the staged units to LDS planes ([row][k], pitch rowb bytes).
-    __device__ __forceinline__ void store(char* lds, int plane_bytes, int rowb, int tid) const
+    __device__ __forceinline__ static void store(const uint32_t* raw, char* lds, int plane_bytes, int rowb, int tid)
     {
 #pragma unroll
         for (int x = 0; x < UPT; ++x) {
@@ -172,23 +174,23 @@ struct Stager {
             if (!active(u)) break;
             int r0, k0;
             unit_coords(u, r0, k0);
+            const uint32_t* r = raw + x * UD;
             // rows[i]: the KU*E bytes of row r0+i (16 bytes, or 8 for int8 M/N-contiguous)
             u32x4 rows[RU];
             if constexpr (KC) {
-                rows[0] = u32x4{raw[x][0], raw[x][1], raw[x][2], raw[x][3]};
+                rows[0] = u32x4{r[0], r[1], r[2], r[3]};
             } else if constexpr (E == 1) {
                 uint32_t o0[4], o1[4];
-                transpose4x4_bytes(&raw[x][0], o0);  // k0..k0+3
-                transpose4x4_bytes(&raw[x][4], o1);  // k0+4..k0+7
+                transpose4x4_bytes(&r[0], o0);  // k0..k0+3
+                transpose4x4_bytes(&r[4], o1);  // k0+4..k0+7
 #pragma unroll
                 for (int i = 0; i < 4; ++i) rows[i] = u32x4{o0[i], o1[i], 0u, 0u};
             } else if constexpr (E == 4) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    rows[i] = u32x4{raw[x][i], raw[x][4 + i], raw[x][8 + i], raw[x][12 + i]};
+                for (int i = 0; i < 4; ++i) rows[i] = u32x4{r[i], r[4 + i], r[8 + i], r[12 + i]};
             } else {  // E == 8: word w = rows r0, r0+1 at k0+w
-                rows[0] = u32x4{raw[x][0], raw[x][1], raw[x][4], raw[x][5]};
-                rows[1] = u32x4{raw[x][2], raw[x][3], raw[x][6], raw[x][7]};
+                rows[0] = u32x4{r[0], r[1], r[4], r[5]};
+                rows[1] = u32x4{r[2], r[3], r[6], r[7]};
             }
             const int kbyte = k0 * AR::LDS_E;
 #pragma unroll
@@ -207,6 +209,45 @@ struct Stager {
                 }
             }
         }
+    }
+};
+
+// Staging registers of one operand.  KCM: 1 = K-contiguous, 0 = M/N-
+// contiguous (fixed at compile time), 2 = chosen per call by the `kc`
+// argument (one register set sized for either layout; used where one
+// kernel serves instances of both layouts).
+template <class AR, int KCM, int ROWS, int BK, int NT>
+struct Stager {
+    using IK = StagerImpl<AR, true, ROWS, BK, NT>;
+    using IN = StagerImpl<AR, false, ROWS, BK, NT>;
+    static constexpr int RAWD = KCM == 1 ? IK::RAWD : KCM == 0 ? IN::RAWD : (IK::RAWD > IN::RAWD ? IK::RAWD : IN::RAWD);
+    uint32_t raw[RAWD];
+
+    __device__ __forceinline__ static bool kc_of(bool kc) { return KCM == 2 ? kc : KCM == 1; }
+    // bytes per global load word for the layout
+    __device__ __forceinline__ static int word_bytes(bool kc) { return kc_of(kc) ? IK::WB : IN::WB; }
+
+    __device__ __forceinline__ void load_fast(bool kc, const char* base, long sr, long sk, int tid)
+    {
+        if constexpr (KCM == 1) IK::load_fast(raw, base, sr, sk, tid);
+        else if constexpr (KCM == 0) IN::load_fast(raw, base, sr, sk, tid);
+        else if (kc) IK::load_fast(raw, base, sr, sk, tid);
+        else IN::load_fast(raw, base, sr, sk, tid);
+    }
+    __device__ __forceinline__ void load_guarded(bool kc, const char* base, long sr, long sk, int rows_valid,
+                                                 int k_valid, bool vec, int tid)
+    {
+        if constexpr (KCM == 1) IK::load_guarded(raw, base, sr, sk, rows_valid, k_valid, vec, tid);
+        else if constexpr (KCM == 0) IN::load_guarded(raw, base, sr, sk, rows_valid, k_valid, vec, tid);
+        else if (kc) IK::load_guarded(raw, base, sr, sk, rows_valid, k_valid, vec, tid);
+        else IN::load_guarded(raw, base, sr, sk, rows_valid, k_valid, vec, tid);
+    }
+    __device__ __forceinline__ void store(bool kc, char* lds, int plane_bytes, int rowb, int tid) const
+    {
+        if constexpr (KCM == 1) IK::store(raw, lds, plane_bytes, rowb, tid);
+        else if constexpr (KCM == 0) IN::store(raw, lds, plane_bytes, rowb, tid);
+        else if (kc) IK::store(raw, lds, plane_bytes, rowb, tid);
+        else IN::store(raw, lds, plane_bytes, rowb, tid);
     }
 };
 
@@ -275,6 +316,7 @@ struct TileJob {
     int i0, j0;     // tile origin in C
     Scalars s;
     int beta0;
+    int akc, bkc;   // operand layouts, read when a tile routine is compiled for either
 };
 
 template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N>
@@ -299,16 +341,20 @@ struct TileShape {
 // is multiplied.  STAGES = 2: two register staging sets, loads issued two
 // k-blocks ahead (covers twice the memory latency for twice the staging
 // registers).
-template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES = 1>
+// AKM / BKM: operand layout mode, 1 = K-contiguous, 0 = M/N-contiguous,
+// 2 = taken from j.akc / j.bkc at run time.
+template <class AR, int AKM, int BKM, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int STAGES = 1>
 __device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
 {
-    using S = TileShape<AR, AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N>;
+    using S = TileShape<AR, true, true, BM, BN, BK, WAVES_M, WAVES_N>;
     using Out = typename AR::Out;
     constexpr int NT = S::NT;
     constexpr int E = AR::E;
     constexpr int KC_L = 16 / AR::LDS_E;  // LDS elements per 16-byte fragment chunk
-    using SA = Stager<AR, AKC, BM, BK, NT>;
-    using SB = Stager<AR, BKC, BN, BK, NT>;
+    using SA = Stager<AR, AKM, BM, BK, NT>;
+    using SB = Stager<AR, BKM, BN, BK, NT>;
+    const bool AKC = AKM == 2 ? (j.akc != 0) : (AKM == 1);
+    const bool BKC = BKM == 2 ? (j.bkc != 0) : (BKM == 1);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -321,8 +367,9 @@ __device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
     const int nk = (j.k + BK - 1) / BK;
     const int rows_a = min(BM, j.m - j.i0);
     const int rows_b = min(BN, j.n - j.j0);
-    const bool a_vec = ((j.lda * E) % SA::WB) == 0 && (reinterpret_cast<uintptr_t>(j.A) % SA::WB) == 0;
-    const bool b_vec = ((j.ldb * E) % SB::WB) == 0 && (reinterpret_cast<uintptr_t>(j.B) % SB::WB) == 0;
+    const int a_wb = SA::word_bytes(AKC), b_wb = SB::word_bytes(BKC);
+    const bool a_vec = ((j.lda * E) % a_wb) == 0 && (reinterpret_cast<uintptr_t>(j.A) % a_wb) == 0;
+    const bool b_vec = ((j.ldb * E) % b_wb) == 0 && (reinterpret_cast<uintptr_t>(j.B) % b_wb) == 0;
     const bool a_fast = a_vec && rows_a == BM;
     const bool b_fast = b_vec && rows_b == BN;
     // tile origin (row i0 / j0, k 0) in bytes
@@ -342,15 +389,15 @@ __device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
         const int kv = j.k - k0;
         const char* ak = Ab + (long)k0 * ska * E;
         const char* bk = Bb + (long)k0 * skb * E;
-        if (a_fast && kv >= BK) xa.load_fast(ak, sra, ska, tid);
-        else xa.load_guarded(ak, sra, ska, rows_a, kv, a_vec, tid);
-        if (b_fast && kv >= BK) xb.load_fast(bk, srb, skb, tid);
-        else xb.load_guarded(bk, srb, skb, rows_b, kv, b_vec, tid);
+        if (a_fast && kv >= BK) xa.load_fast(AKC, ak, sra, ska, tid);
+        else xa.load_guarded(AKC, ak, sra, ska, rows_a, kv, a_vec, tid);
+        if (b_fast && kv >= BK) xb.load_fast(BKC, bk, srb, skb, tid);
+        else xb.load_guarded(BKC, bk, srb, skb, rows_b, kv, b_vec, tid);
     };
     auto store_step = [&](int buf, const SA& xa, const SB& xb) {
         char* base = smem + buf * S::BUF;
-        xa.store(base, S::A_PLANE, S::ROWB, tid);
-        xb.store(base + AR::NPLANE * S::A_PLANE, S::B_PLANE, S::ROWB, tid);
+        xa.store(AKC, base, S::A_PLANE, S::ROWB, tid);
+        xb.store(BKC, base + AR::NPLANE * S::A_PLANE, S::B_PLANE, S::ROWB, tid);
     };
     auto compute = [&](int buf) {
         const char* la = smem + buf * S::BUF;
@@ -443,6 +490,8 @@ __device__ __forceinline__ TileJob make_job(const KParams& p, int inst, int i0, 
     j.j0 = j0;
     j.s = p.s;
     j.beta0 = p.beta0;
+    j.akc = 0;
+    j.bkc = 0;
     return j;
 }
 
@@ -459,7 +508,7 @@ tile_gemm(const KParams p)
         const int inst = (int)(w / tiles_per_inst);
         const int rem = (int)(w % tiles_per_inst);
         const TileJob j = make_job(p, inst, (rem % p.tiles_m) * BM, (rem / p.tiles_m) * BN);
-        tile_job<AR, AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, STAGES>(j, smem);
+        tile_job<AR, AKC ? 1 : 0, BKC ? 1 : 0, BM, BN, BK, WAVES_M, WAVES_N, STAGES>(j, smem);
     }
 }
 

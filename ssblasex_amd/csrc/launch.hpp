@@ -33,7 +33,13 @@ enum TileConfig : int {
     CFG_TILE_128_K2_S2 = 11,
     CFG_TILE_128_W8_S2 = 12,
     CFG_TILE_256x128_S2 = 13,
-    CFG_COUNT = 14
+    // int8 only (other forms use TILE_128): direct-to-LDS loads, S stages
+    CFG_DMA_64 = 14,        // 64x64, k 64, 4 stages
+    CFG_DMA_128 = 15,       // 128x128, k 64, 4 stages
+    CFG_DMA_128_K2 = 16,    // 128x128, k 128, 3 stages
+    CFG_DMA_256x128 = 17,   // 256x128, 8 waves, k 64, 3 stages
+    CFG_DMA_128_S3 = 18,    // 128x128, k 64, 3 stages
+    CFG_COUNT = 19
 };
 
 const char* config_name(int cfg);

@@ -2,6 +2,7 @@
 // every layout and tile config, and sizes the persistent grid from the
 // kernel's occupancy.  Included by exactly one .hip file per form.
 #pragma once
+#include "gemm_dma.hpp"
 #include "gemm_kernels.hpp"
 #include "launch.hpp"
 
@@ -93,6 +94,24 @@ hipError_t run_tile(KParams p, hipStream_t st)
     }
 }
 
+template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WM, int WN, int S>
+hipError_t run_dma(KParams p, hipStream_t st)
+{
+    if constexpr (AR::ID != ARITH_I8) {
+        return run_tile<AR, AKC, BKC, CFG_TILE_128>(p, st);  // direct-to-LDS tiles exist for int8 only
+    } else {
+        constexpr int NT = 64 * WM * WN;
+        auto kern = dma_gemm_i8<AKC, BKC, BM, BN, BK, WM, WN, S>;
+        p.tiles_m = (p.m + BM - 1) / BM;
+        p.tiles_n = (p.n + BN - 1) / BN;
+        p.total = (long)p.batch * p.tiles_m * p.tiles_n;
+        int occ = OccCache<decltype(kern)>::get(kern, NT);
+        long grid = persistent_grid(p.total, occ);
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), 0, st, p);
+        return hipGetLastError();
+    }
+}
+
 template <class AR, bool AKC, bool BKC, int W>
 hipError_t run_direct(KParams p, hipStream_t st)
 {
@@ -125,6 +144,11 @@ hipError_t run_cfg(const KParams& p, int cfg, hipStream_t st)
     case CFG_TILE_128_K2_S2: return run_tile<AR, AKC, BKC, CFG_TILE_128_K2_S2>(p, st);
     case CFG_TILE_128_W8_S2: return run_tile<AR, AKC, BKC, CFG_TILE_128_W8_S2>(p, st);
     case CFG_TILE_256x128_S2: return run_tile<AR, AKC, BKC, CFG_TILE_256x128_S2>(p, st);
+    case CFG_DMA_64: return run_dma<AR, AKC, BKC, 64, 64, 64, 2, 2, 4>(p, st);
+    case CFG_DMA_128: return run_dma<AR, AKC, BKC, 128, 128, 64, 2, 2, 4>(p, st);
+    case CFG_DMA_128_K2: return run_dma<AR, AKC, BKC, 128, 128, 128, 2, 2, 3>(p, st);
+    case CFG_DMA_256x128: return run_dma<AR, AKC, BKC, 256, 128, 64, 4, 2, 3>(p, st);
+    case CFG_DMA_128_S3: return run_dma<AR, AKC, BKC, 128, 128, 64, 2, 2, 3>(p, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -179,10 +179,10 @@ __global__ void __launch_bounds__(256) ozaki_gemm(const OzParams p)
     const int tiles_per_inst = p.tiles_m * p.tiles_n;
     const int s = p.s;
 
-    Stager<AR, AKC, BM, BK, NT> sa;
-    Stager<AR, BKC, BN, BK, NT> sb;
-    using SA = Stager<AR, AKC, BM, BK, NT>;
-    using SB = Stager<AR, BKC, BN, BK, NT>;
+    using SA = Stager<AR, AKC ? 1 : 0, BM, BK, NT>;
+    using SB = Stager<AR, BKC ? 1 : 0, BN, BK, NT>;
+    SA sa;
+    SB sb;
 
     for (long t = blockIdx.x; t < p.total; t += gridDim.x) {
         const long w = xcd_remap(t, p.total);
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) ozaki_gemm(const OzParams p)
         const int rows_a = min(BM, p.m - i0), rows_b = min(BN, p.n - j0);
         const char* Ai = reinterpret_cast<const char*>(p.wa) + (long)inst * s * plane_a + (long)i0 * sra;
         const char* Bi = reinterpret_cast<const char*>(p.wb) + (long)inst * s * plane_b + (long)j0 * srb;
-        const bool a_vec = (p.ld_sa % SA::WB) == 0, b_vec = (p.ld_sb % SB::WB) == 0;
+        const bool a_vec = (p.ld_sa % SA::word_bytes(AKC)) == 0, b_vec = (p.ld_sb % SB::word_bytes(BKC)) == 0;
         const bool a_fast = a_vec && rows_a == BM, b_fast = b_vec && rows_b == BN;
 
         f64x4 D[FM][FN];
@@ -221,15 +221,15 @@ __global__ void __launch_bounds__(256) ozaki_gemm(const OzParams p)
                     const char* ak = Ai + (long)(pa - 1) * plane_a + (long)k0 * ska;
                     const char* bk = Bi + (long)(pb - 1) * plane_b + (long)k0 * skb;
                     const int kv_c = min(kv, kc0 + kc - k0);
-                    if (a_fast && kv_c >= BK) sa.load_fast(ak, sra, ska, tid);
-                    else sa.load_guarded(ak, sra, ska, rows_a, kv_c, a_vec, tid);
-                    if (b_fast && kv_c >= BK) sb.load_fast(bk, srb, skb, tid);
-                    else sb.load_guarded(bk, srb, skb, rows_b, kv_c, b_vec, tid);
+                    if (a_fast && kv_c >= BK) sa.load_fast(AKC, ak, sra, ska, tid);
+                    else sa.load_guarded(AKC, ak, sra, ska, rows_a, kv_c, a_vec, tid);
+                    if (b_fast && kv_c >= BK) sb.load_fast(BKC, bk, srb, skb, tid);
+                    else sb.load_guarded(BKC, bk, srb, skb, rows_b, kv_c, b_vec, tid);
                 };
                 auto store_step = [&](int buf) {
                     char* base = smem + buf * BUF;
-                    sa.store(base, A_PLANE, ROWB, tid);
-                    sb.store(base + A_PLANE, B_PLANE, ROWB, tid);
+                    sa.store(AKC, base, A_PLANE, ROWB, tid);
+                    sb.store(BKC, base + A_PLANE, B_PLANE, ROWB, tid);
                 };
 
                 load_step(0);

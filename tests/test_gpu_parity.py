@@ -28,7 +28,8 @@ NTHREADS = min(16, os.cpu_count() or 4)
 SHAPES = [(1, 1, 1), (7, 5, 3), (16, 16, 64), (33, 17, 65), (64, 64, 64), (65, 63, 130), (128, 128, 128),
           (130, 257, 200)]
 CONFIGS = ["direct16", "direct32", "tile64", "tile128", "tile128x64", "tile64x128", "tile128k2", "tile128w8",
-           "tile256x128", "tile64s2", "tile128s2", "tile128k2s2", "tile128w8s2", "tile256x128s2"]
+           "tile256x128", "tile64s2", "tile128s2", "tile128k2s2", "tile128w8s2", "tile256x128s2", "dma64",
+           "dma128", "dma128k2", "dma256x128", "dma128s3"]
 
 
 @pytest.fixture(autouse=True)
@@ -98,6 +99,17 @@ def test_every_tile_config(cfg, t, ta, tb):
     alpha, beta = (2, 1) if t == "i8" else (0.75, 1.25)
     check(make_batch(t, ta, tb, 150, 97, 203, 2, alpha=alpha, beta=beta, pad=(3, 0, 5), seed=17))
     check(make_batch(t, ta, tb, 128, 128, 128, 2, alpha=alpha, beta=0, seed=18))
+
+
+@pytest.mark.parametrize("cfg", ["dma64", "dma128", "dma128k2", "dma256x128", "dma128s3"])
+@pytest.mark.parametrize("ta,tb", OPS)
+def test_int8_dma_tiles(cfg, ta, tb):
+    """Direct-to-LDS int8 tiles on whole tiles (DMA path) and on a ragged
+    shape (guarded path), beta 0 and 1."""
+    os.environ["SSBLAS_TILE_CONFIG"] = cfg
+    check(make_batch("i8", ta, tb, 256, 256, 256, 3, alpha=1, beta=0, seed=61))
+    check(make_batch("i8", ta, tb, 256, 256, 384, 2, alpha=-2, beta=1, seed=62))
+    check(make_batch("i8", ta, tb, 200, 136, 100, 2, alpha=3, beta=-1, pad=(1, 0, 2), seed=63))
 
 
 @pytest.mark.parametrize("t", ["i8", "f32", "f64"])
