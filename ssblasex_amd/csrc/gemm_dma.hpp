@@ -124,97 +124,85 @@ __device__ __forceinline__ void lds_barrier()
     __asm__ __volatile__("" ::: "memory");
 }
 
+// ---------------------------------------------------------------------------
+// Building blocks shared by the per-tile and the streaming kernels
+// ---------------------------------------------------------------------------
 template <bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int S>
-__device__ __forceinline__ void dma_tile_job(const TileJob& j, char* smem)
-{
+struct DmaTile {
     using SH = DmaShape<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>;
     using OA = typename SH::OA;
     using OB = typename SH::OB;
     using AR = ArithI8;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
-    const int nk = (j.k + BK - 1) / BK;
-    const int rows_a = min(BM, j.m - j.i0), rows_b = min(BN, j.n - j.j0);
-    const bool dma = rows_a == BM && rows_b == BN && (j.k % BK) == 0 && (j.lda % 16) == 0 &&
-                     (j.ldb % 16) == 0 && aligned16(j.A) && aligned16(j.B);
 
-    // ---- stage issue: DMA (whole tile in range) or guarded register path
-    auto issue = [&](int kt, int buf) {
-        char* base = smem + buf * SH::STAGE;
+    // The whole tile and every k-block lie inside aligned matrices.
+    __device__ __forceinline__ static bool eligible(const TileJob& j)
+    {
+        return j.i0 + BM <= j.m && j.j0 + BN <= j.n && (j.k % BK) == 0 && (j.lda % 16) == 0 && (j.ldb % 16) == 0 &&
+               aligned16(j.A) && aligned16(j.B);
+    }
+
+    // DMA k-block kt of an eligible tile into stage buffer `base`.
+    __device__ __forceinline__ static void issue_dma(const TileJob& j, int kt, char* base, int wave, int lane)
+    {
         const int k0 = kt * BK;
-        if (dma) {
 #pragma unroll
-            for (int q = 0; q < SH::PER_WAVE; ++q) {
-                const int inst = wave + q * SH::NWAVE;       // wave instruction index in the stage
-                const int slot = inst * 64 + lane;           // chunk slot written by this lane
-                const char* src;
-                if (slot < OA::CHUNKS) {
-                    const int row = slot / OA::CP, c = (slot % OA::CP) ^ dma_swz<OA::CP>(row);
-                    src = j.A + OA::chunk_off(row, c, j.i0, k0, j.lda);
-                } else {
-                    const int s2 = slot - OA::CHUNKS;
-                    const int row = s2 / OB::CP, c = (s2 % OB::CP) ^ dma_swz<OB::CP>(row);
-                    src = j.B + OB::chunk_off(row, c, j.j0, k0, j.ldb);
-                }
-                __builtin_amdgcn_global_load_lds(src, (lds_void_t*)(base + inst * 1024), 16, 0, 0);
+        for (int q = 0; q < SH::PER_WAVE; ++q) {
+            const int inst = wave + q * SH::NWAVE;  // wave instruction index in the stage
+            const int slot = inst * 64 + lane;      // chunk slot written by this lane
+            const char* src;
+            if (slot < OA::CHUNKS) {
+                const int row = slot / OA::CP, c = (slot % OA::CP) ^ dma_swz<OA::CP>(row);
+                src = j.A + OA::chunk_off(row, c, j.i0, k0, j.lda);
+            } else {
+                const int s2 = slot - OA::CHUNKS;
+                const int row = s2 / OB::CP, c = (s2 % OB::CP) ^ dma_swz<OB::CP>(row);
+                src = j.B + OB::chunk_off(row, c, j.j0, k0, j.ldb);
             }
-        } else {
-            const int kv = j.k - k0;
-            for (int slot = tid; slot < SH::CHUNKS; slot += SH::NT) {
-                const bool isa = slot < OA::CHUNKS;
-                const int s2 = isa ? slot : slot - OA::CHUNKS;
-                const int cp = isa ? OA::CP : OB::CP;
-                const int row = s2 / cp;
-                const int c = (s2 % cp) ^ (isa ? dma_swz<OA::CP>(row) : dma_swz<OB::CP>(row));
-                const bool kc = isa ? AKC : BKC;
-                const char* X = isa ? j.A : j.B;
-                const long ld = isa ? j.lda : j.ldb;
-                const int mn0 = isa ? j.i0 : j.j0;
-                const int mnv = isa ? rows_a : rows_b;
-                // elements e = 0..15 of the chunk: KC -> (mn = row, k = 16c+e); MN -> (k = row, mn = 16c+e)
-                uint32_t w[4] = {0u, 0u, 0u, 0u};
-                const long off0 = isa ? OA::chunk_off(row, c, mn0, k0, ld) : OB::chunk_off(row, c, mn0, k0, ld);
-                bool whole, in_row;
-                if (kc) { in_row = row < mnv; whole = in_row && 16 * c + 16 <= kv; }
-                else { in_row = row < kv; whole = in_row && 16 * c + 16 <= mnv; }
-                if (whole && (reinterpret_cast<uintptr_t>(X + off0) & 15) == 0) {
-                    u32x4 v = *reinterpret_cast<const u32x4*>(X + off0);
-                    w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
-                } else if (in_row) {
+            __builtin_amdgcn_global_load_lds(src, (lds_void_t*)(base + inst * 1024), 16, 0, 0);
+        }
+    }
+
+    // Guarded register path: same LDS layout, zero outside the matrices.
+    __device__ __forceinline__ static void issue_regs(const TileJob& j, int kt, char* base, int tid)
+    {
+        const int k0 = kt * BK;
+        const int kv = j.k - k0;
+        const int rows_a = min(BM, j.m - j.i0), rows_b = min(BN, j.n - j.j0);
+        for (int slot = tid; slot < SH::CHUNKS; slot += SH::NT) {
+            const bool isa = slot < OA::CHUNKS;
+            const int s2 = isa ? slot : slot - OA::CHUNKS;
+            const int cp = isa ? OA::CP : OB::CP;
+            const int row = s2 / cp;
+            const int c = (s2 % cp) ^ (isa ? dma_swz<OA::CP>(row) : dma_swz<OB::CP>(row));
+            const bool kc = isa ? AKC : BKC;
+            const char* X = isa ? j.A : j.B;
+            const long ld = isa ? j.lda : j.ldb;
+            const int mn0 = isa ? j.i0 : j.j0;
+            const int mnv = isa ? rows_a : rows_b;
+            // elements e = 0..15 of the chunk: KC -> (mn = row, k = 16c+e); MN -> (k = row, mn = 16c+e)
+            uint32_t w[4] = {0u, 0u, 0u, 0u};
+            const long off0 = isa ? OA::chunk_off(row, c, mn0, k0, ld) : OB::chunk_off(row, c, mn0, k0, ld);
+            bool whole, in_row;
+            if (kc) { in_row = row < mnv; whole = in_row && 16 * c + 16 <= kv; }
+            else { in_row = row < kv; whole = in_row && 16 * c + 16 <= mnv; }
+            if (whole && (reinterpret_cast<uintptr_t>(X + off0) & 15) == 0) {
+                u32x4 v = *reinterpret_cast<const u32x4*>(X + off0);
+                w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+            } else if (in_row) {
 #pragma unroll
-                    for (int e = 0; e < 16; ++e) {
-                        const bool ok = kc ? (16 * c + e < kv) : (16 * c + e < mnv);
-                        const uint32_t b = ok ? (uint32_t)(uint8_t)X[off0 + e] : 0u;
-                        w[e >> 2] |= b << (8 * (e & 3));
-                    }
+                for (int e = 0; e < 16; ++e) {
+                    const bool ok = kc ? (16 * c + e < kv) : (16 * c + e < mnv);
+                    const uint32_t b = ok ? (uint32_t)(uint8_t)X[off0 + e] : 0u;
+                    w[e >> 2] |= b << (8 * (e & 3));
                 }
-                *reinterpret_cast<u32x4*>(smem + buf * SH::STAGE + slot * 16) = u32x4{w[0], w[1], w[2], w[3]};
             }
+            *reinterpret_cast<u32x4*>(base + slot * 16) = u32x4{w[0], w[1], w[2], w[3]};
         }
-    };
+    }
 
-    i32x4 acc[SH::FM][SH::FN];
-#pragma unroll
-    for (int fm = 0; fm < SH::FM; ++fm)
-#pragma unroll
-        for (int fn = 0; fn < SH::FN; ++fn) acc[fm][fn] = i32x4{0, 0, 0, 0};
-
-    lds_barrier();  // previous user of smem has finished reading it
-#pragma unroll
-    for (int s = 0; s < S - 1; ++s)
-        if (s < nk) issue(s, s);
-
-    for (int kt = 0; kt < nk; ++kt) {
-        // block kt complete for this wave (later blocks may stay in flight)
-        if (dma) {
-            if (kt + S - 2 < nk) wait_vm<(S - 2) * SH::PER_WAVE>();
-            else wait_vm<0>();
-        } else {
-            wait_lgkm0();
-        }
-        lds_barrier();  // ... and for every wave; buffer (kt-1)%S is free
-        if (kt + S - 1 < nk) issue(kt + S - 1, (kt + S - 1) % S);
-        const char* la = smem + (kt % S) * SH::STAGE;
+    __device__ __forceinline__ static void compute(i32x4 (&acc)[SH::FM][SH::FN], const char* la, int wm, int wn,
+                                                   int lane)
+    {
         const char* lb = la + OA::BYTES;
 #pragma unroll
         for (int ks = 0; ks < BK / 64; ++ks) {
@@ -230,16 +218,75 @@ __device__ __forceinline__ void dma_tile_job(const TileJob& j, char* smem)
         }
     }
 
-    int32_t* C = static_cast<int32_t*>(j.C);
-    const bool c_fast = rows_a == BM && rows_b == BN && ((j.ldc * 4) % 16) == 0 && aligned16(C);
+    __device__ __forceinline__ static void epilogue(const TileJob& j, i32x4 (&acc)[SH::FM][SH::FN], int wm, int wn,
+                                                    int lane)
+    {
+        int32_t* C = static_cast<int32_t*>(j.C);
+        const bool c_fast = j.i0 + BM <= j.m && j.j0 + BN <= j.n && ((j.ldc * 4) % 16) == 0 && aligned16(C);
 #pragma unroll
-    for (int fm = 0; fm < SH::FM; ++fm)
+        for (int fm = 0; fm < SH::FM; ++fm)
 #pragma unroll
-        for (int fn = 0; fn < SH::FN; ++fn) {
-            const int r0 = j.i0 + wm * SH::WTM + fm * 16 + 4 * (lane >> 4);
-            const int c = j.j0 + wn * SH::WTN + fn * 16 + (lane & 15);
-            store_frag<AR>(C, j.ldc, r0, c, acc[fm][fn], j.s, j.beta0 != 0, c_fast, j.m, j.n);
+            for (int fn = 0; fn < SH::FN; ++fn) {
+                const int r0 = j.i0 + wm * SH::WTM + fm * 16 + 4 * (lane >> 4);
+                const int c = j.j0 + wn * SH::WTN + fn * 16 + (lane & 15);
+                store_frag<AR>(C, j.ldc, r0, c, acc[fm][fn], j.s, j.beta0 != 0, c_fast, j.m, j.n);
+            }
+    }
+
+    __device__ __forceinline__ static void zero(i32x4 (&acc)[SH::FM][SH::FN])
+    {
+#pragma unroll
+        for (int fm = 0; fm < SH::FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < SH::FN; ++fn) acc[fm][fn] = i32x4{0, 0, 0, 0};
+    }
+
+    // Wait until at most `ahead` stages beyond the one about to be used are
+    // still in flight (DMA), then barrier.
+    __device__ __forceinline__ static void wait_stage(int ahead)
+    {
+        if (ahead >= S - 2) wait_vm<(S - 2) * SH::PER_WAVE>();
+        else wait_vm<0>();
+        lds_barrier();
+    }
+};
+
+// One tile, S-stage pipeline inside the tile (the previous tile's buffers
+// must be free: the routine starts with a barrier).
+template <bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int S>
+__device__ __forceinline__ void dma_tile_job(const TileJob& j, char* smem)
+{
+    using T = DmaTile<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>;
+    using SH = typename T::SH;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+    const int nk = (j.k + BK - 1) / BK;
+    const bool dma = T::eligible(j);
+
+    i32x4 acc[SH::FM][SH::FN];
+    T::zero(acc);
+    lds_barrier();  // previous user of smem has finished reading it
+#pragma unroll
+    for (int s = 0; s < S - 1; ++s) {
+        if (s < nk) {
+            if (dma) T::issue_dma(j, s, smem + s * SH::STAGE, wave, lane);
+            else T::issue_regs(j, s, smem + s * SH::STAGE, tid);
         }
+    }
+    for (int kt = 0; kt < nk; ++kt) {
+        if (dma) {
+            T::wait_stage(min(nk - 1 - kt, S - 2));
+        } else {
+            wait_lgkm0();
+            lds_barrier();
+        }
+        if (kt + S - 1 < nk) {
+            if (dma) T::issue_dma(j, kt + S - 1, smem + ((kt + S - 1) % S) * SH::STAGE, wave, lane);
+            else T::issue_regs(j, kt + S - 1, smem + ((kt + S - 1) % S) * SH::STAGE, tid);
+        }
+        T::compute(acc, smem + (kt % S) * SH::STAGE, wm, wn, lane);
+    }
+    T::epilogue(j, acc, wm, wn, lane);
 }
 
 template <bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int S>
@@ -255,6 +302,88 @@ dma_gemm_i8(const KParams p)
         const int rem = (int)(w % tiles_per_inst);
         const TileJob j = make_job(p, inst, (rem % p.tiles_m) * BM, (rem / p.tiles_m) * BN);
         dma_tile_job<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>(j, smem);
+    }
+}
+
+// Streaming kernel for problems whose tiles are all whole (m, n, k multiples
+// of the tile, leading dimensions multiples of 16): the k-blocks of all the
+// workgroup's tiles form one stream, so the next tile's first blocks are in
+// flight while the current tile is finished and its C written.  A tile
+// whose instance pointers are not 16-byte aligned is run on its own
+// (guarded path) after the stream drains, and the stream restarts behind it.
+template <bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int S>
+__global__ void __launch_bounds__(64 * WAVES_M * WAVES_N)
+dma_stream_gemm_i8(const KParams p)
+{
+    using T = DmaTile<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>;
+    using SH = typename T::SH;
+    __shared__ __attribute__((aligned(1024))) char smem[SH::LDS_BYTES];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+    const int tiles_per_inst = p.tiles_m * p.tiles_n;
+    const int nk = p.k / BK;
+    const long G = gridDim.x;
+    auto job_of = [&](long t) {
+        const long w = xcd_remap(t, p.total);
+        const int inst = (int)(w / tiles_per_inst);
+        const int rem = (int)(w % tiles_per_inst);
+        return make_job(p, inst, (rem % p.tiles_m) * BM, (rem / p.tiles_m) * BN);
+    };
+
+    long t_c = blockIdx.x;  // tile being computed
+    if (t_c >= p.total) return;
+    TileJob cur = job_of(t_c);
+    // issue cursor: tile t_i, k-block k_i, stream position u_i
+    long t_i = t_c;
+    int k_i = 0;
+    TileJob ij = cur;
+    bool i_ok = T::eligible(ij);
+    long u_i = 0, u_c = 0;
+    auto issue_next = [&]() {
+        if (t_i >= p.total || !i_ok) return false;
+        T::issue_dma(ij, k_i, smem + (u_i % S) * SH::STAGE, wave, lane);
+        ++u_i;
+        if (++k_i == nk) {
+            k_i = 0;
+            t_i += G;
+            if (t_i < p.total) {
+                ij = job_of(t_i);
+                i_ok = T::eligible(ij);
+            }
+        }
+        return true;
+    };
+
+    i32x4 acc[SH::FM][SH::FN];
+    while (t_c < p.total) {
+        if (!T::eligible(cur)) {
+            // nothing of this tile was issued (the stream stops in front of it)
+            wait_vm<0>();
+            dma_tile_job<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>(cur, smem);
+            t_c += G;
+            if (t_c >= p.total) break;
+            cur = job_of(t_c);
+            t_i = t_c;
+            k_i = 0;
+            ij = cur;
+            i_ok = T::eligible(ij);
+            u_c = u_i = 0;
+            lds_barrier();
+            continue;
+        }
+        while (u_i < u_c + S - 1 && issue_next()) {
+        }
+        T::zero(acc);
+        for (int kt = 0; kt < nk; ++kt) {
+            T::wait_stage((int)min((long)(S - 2), u_i - u_c - 1));
+            while (u_i < u_c + S && issue_next()) {
+            }
+            T::compute(acc, smem + (u_c % S) * SH::STAGE, wm, wn, lane);
+            ++u_c;
+        }
+        T::epilogue(cur, acc, wm, wn, lane);
+        t_c += G;
+        if (t_c < p.total) cur = job_of(t_c);
     }
 }
 

@@ -2,6 +2,7 @@
 // every layout and tile config, and sizes the persistent grid from the
 // kernel's occupancy.  Included by exactly one .hip file per form.
 #pragma once
+#include <stdlib.h>
 #include "gemm_dma.hpp"
 #include "gemm_kernels.hpp"
 #include "launch.hpp"
@@ -101,13 +102,21 @@ hipError_t run_dma(KParams p, hipStream_t st)
         return run_tile<AR, AKC, BKC, CFG_TILE_128>(p, st);  // direct-to-LDS tiles exist for int8 only
     } else {
         constexpr int NT = 64 * WM * WN;
-        auto kern = dma_gemm_i8<AKC, BKC, BM, BN, BK, WM, WN, S>;
         p.tiles_m = (p.m + BM - 1) / BM;
         p.tiles_n = (p.n + BN - 1) / BN;
         p.total = (long)p.batch * p.tiles_m * p.tiles_n;
-        int occ = OccCache<decltype(kern)>::get(kern, NT);
-        long grid = persistent_grid(p.total, occ);
-        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), 0, st, p);
+        const bool whole = p.m % BM == 0 && p.n % BN == 0 && p.k % BK == 0 && p.lda % 16 == 0 && p.ldb % 16 == 0;
+        if (whole && getenv("SSBLAS_NO_STREAM") == nullptr) {
+            auto kern = dma_stream_gemm_i8<AKC, BKC, BM, BN, BK, WM, WN, S>;
+            int occ = OccCache<decltype(kern)>::get(kern, NT);
+            long grid = persistent_grid(p.total, occ);
+            hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), 0, st, p);
+        } else {
+            auto kern = dma_gemm_i8<AKC, BKC, BM, BN, BK, WM, WN, S>;
+            int occ = OccCache<decltype(kern)>::get(kern, NT);
+            long grid = persistent_grid(p.total, occ);
+            hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), 0, st, p);
+        }
         return hipGetLastError();
     }
 }
