@@ -10,11 +10,12 @@ inputs already in HBM.  W untimed warmup steps, then K steps bracketed by
 HIP events on the launch stream and device synchronisation.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--beta 0|1]
-                    [--no-cpu-baseline] [--secondary]
+                    [--no-cpu-baseline] [--secondary] [--grouped]
 
 Prints one JSON line.  --secondary adds the other benchmark configurations
 (fp32 64^3 x128 NN, fp64 32^3 x65536 NT native vs Ozaki, fp32 256^3 x4096
-TN native vs BF16x3/x6) under "secondary".
+TN native vs BF16x3/x6) under "secondary"; --grouped (implied by
+--secondary) adds the grouped 10000-instance uneven batch under "grouped_c5".
 """
 from __future__ import annotations
 
@@ -40,6 +41,7 @@ def parse():
     ap.add_argument("--beta", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--secondary", action="store_true")
+    ap.add_argument("--grouped", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
@@ -212,6 +214,95 @@ def secondary(args, stream):
     return out
 
 
+def grouped_c5(args, stream):
+    """Grouped uneven batch: 10000 instances, m, n, k = floor(exp(U[ln 8, ln 513))),
+    ld = dim + U{0..16}, transa/transb drawn per group of 256, groups alternately
+    fp32 N(0,1) and int8 U{-128..127}, alpha = 1, beta = 0, buffers allocated in a
+    shuffled order.  One step = one replay of the plan's captured graph.
+    Also times the instances of each size class on their own (plans over the
+    subsets), which shows the memory-bound small end and the MFMA-bound large end."""
+    import math
+
+    import torch
+
+    import ssblasex_amd as S
+
+    rng = np.random.default_rng(SEED + 5)
+    count, group = 10000, 256
+    ngroups = (count + group - 1) // group
+    gsize = [min(group, count - g * group) for g in range(ngroups)]
+    gtype = ["f32" if g % 2 == 0 else "i8" for g in range(ngroups)]
+    gta = [int(x) for x in rng.integers(0, 2, ngroups)]
+    gtb = [int(x) for x in rng.integers(0, 2, ngroups)]
+    dims = np.floor(np.exp(rng.uniform(math.log(8), math.log(513), (count, 3)))).astype(np.int64)
+    pads = rng.integers(0, 17, (count, 3))
+    inst = []  # (type, ta, tb, m, n, k, lda, ldb, ldc)
+    for g in range(ngroups):
+        for i in range(g * group, g * group + gsize[g]):
+            m, n, k = (int(x) for x in dims[i])
+            ra, rb = (k if gta[g] else m), (n if gtb[g] else k)
+            inst.append((gtype[g], gta[g], gtb[g], m, n, k, ra + int(pads[i, 0]), rb + int(pads[i, 1]),
+                         m + int(pads[i, 2])))
+    gen = torch.Generator(device="cuda").manual_seed(SEED + 5)
+    bufs = [None] * (3 * count)
+    for j in rng.permutation(3 * count):
+        i, w = divmod(int(j), 3)
+        t, ta, tb, m, n, k, lda, ldb, ldc = inst[i]
+        cols = ((m if ta else k), (k if tb else n), n)[w]
+        size = (lda, ldb, ldc)[w] * cols
+        if t == "f32":
+            bufs[j] = torch.randn(size, device="cuda", generator=gen)
+        elif w < 2:
+            bufs[j] = torch.randint(-128, 128, (size,), device="cuda", dtype=torch.int8, generator=gen)
+        else:
+            bufs[j] = torch.zeros(size, device="cuda", dtype=torch.int32)
+    codes = {"f32": (S.DataType.R_32F, S.DataType.R_32F, 1.0, 0.0), "i8": (S.DataType.R_8I, S.DataType.R_32I, 1, 0)}
+
+    def run(sel):
+        """Build a plan over the instances in `sel` (kept in group order) and time its graph."""
+        groups = []  # (type, ta, tb, [instances])
+        for i in sel:
+            t, ta, tb = inst[i][:3]
+            g = i // group
+            if not groups or groups[-1][0] != g:
+                groups.append((g, t, ta, tb, []))
+            groups[-1][4].append(i)
+        ids = [i for grp in groups for i in grp[4]]
+        pa = S.pointer_array([bufs[3 * i] for i in ids])
+        pb = S.pointer_array([bufs[3 * i + 1] for i in ids])
+        pc = S.pointer_array([bufs[3 * i + 2] for i in ids])
+        plan = S.GroupedPlan([len(g[4]) for g in groups], [g[2] for g in groups], [g[3] for g in groups],
+                             [codes[g[1]][0] for g in groups], [codes[g[1]][0] for g in groups],
+                             [codes[g[1]][1] for g in groups], [codes[g[1]][2] for g in groups],
+                             [codes[g[1]][3] for g in groups], [inst[i][3] for i in ids], [inst[i][4] for i in ids],
+                             [inst[i][5] for i in ids], pa, [inst[i][6] for i in ids], pb, [inst[i][7] for i in ids],
+                             pc, [inst[i][8] for i in ids])
+        ms = time_steps(lambda: plan.graph_launch(stream), max(5, args.steps // 10), 3, stream)
+        plan.close()
+        flops = {"f32": 0, "i8": 0}
+        nbytes = 0
+        for i in ids:
+            t, ta, tb, m, n, k = inst[i][:6]
+            flops[t] += 2 * m * n * k
+            e = 4 if t == "f32" else 1
+            nbytes += (m * k + k * n) * e + m * n * 4 + 24
+        tot = flops["f32"] + flops["i8"]
+        return {"instances": len(ids), "ms_per_step": round(ms, 4), "GFLOPs_total": round(tot / (ms * 1e-3) / 1e9, 1),
+                "fp32_GFLOP": round(flops["f32"] / 1e9, 3), "int8_GOP": round(flops["i8"] / 1e9, 3),
+                "hbm_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "fp32_mfma_frac_if_fp32_alone": round(flops["f32"] / (ms * 1e-3) / 1e12 / PEAK["f32"], 4)}
+
+    out = {"all": run(range(count))}
+    big = [i for i in range(count) if min(inst[i][3:6]) >= 256]
+    small = [i for i in range(count) if max(inst[i][3:6]) <= 32]
+    out["small (max dim <= 32)"] = run(small)
+    out["large (min dim >= 256)"] = run(big)
+    del bufs
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -259,6 +350,8 @@ def main():
         line["cpu_baseline"] = cpu_baseline_i8(prob, threads)
     if args.secondary:
         line["secondary"] = secondary(args, stream)
+    if args.grouped or args.secondary:
+        line["grouped_c5"] = grouped_c5(args, stream)
     print(json.dumps(line), flush=True)
 
 

@@ -333,6 +333,38 @@ struct TileShape {
     static_assert(BK % AR::KSTEP == 0, "BK must hold whole MFMA k-steps");
 };
 
+// MFMA work of one LDS k-block: the wave's FM x FN fragments over BK.
+// la / lb: the A and B planes of the k-block ([row][k], pitch S::ROWB).
+template <class AR, class S, int BK>
+__device__ __forceinline__ void tile_compute(typename AR::Acc (&acc)[S::FM][S::FN], const char* la, const char* lb,
+                                             int wm, int wn, int lane)
+{
+    constexpr int KC_L = 16 / AR::LDS_E;  // LDS elements per 16-byte fragment chunk
+#pragma unroll
+    for (int ks = 0; ks < BK / AR::KSTEP; ++ks) {
+        const int kbyte = (ks * AR::KSTEP + (lane >> 4) * KC_L) * AR::LDS_E;
+        u32x4 af[S::FM][AR::NPLANE], bfg[S::FN][AR::NPLANE];
+#pragma unroll
+        for (int fm = 0; fm < S::FM; ++fm) {
+            const int row = wm * S::WTM + fm * 16 + frag_row<AR>(lane & 15);
+#pragma unroll
+            for (int pl = 0; pl < AR::NPLANE; ++pl)
+                af[fm][pl] = *reinterpret_cast<const u32x4*>(la + pl * S::A_PLANE + row * S::ROWB + kbyte);
+        }
+#pragma unroll
+        for (int fn = 0; fn < S::FN; ++fn) {
+            const int row = wn * S::WTN + fn * 16 + (lane & 15);
+#pragma unroll
+            for (int pl = 0; pl < AR::NPLANE; ++pl)
+                bfg[fn][pl] = *reinterpret_cast<const u32x4*>(lb + pl * S::B_PLANE + row * S::ROWB + kbyte);
+        }
+#pragma unroll
+        for (int fm = 0; fm < S::FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < S::FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
+    }
+}
+
 // LDS-staged workgroup tile: BM x BN of C, all threads of the workgroup.
 // smem must hold TileShape::LDS_BYTES; the caller's previous use of it must
 // be finished by every thread (the routine starts with a barrier).
@@ -350,7 +382,6 @@ __device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
     using Out = typename AR::Out;
     constexpr int NT = S::NT;
     constexpr int E = AR::E;
-    constexpr int KC_L = 16 / AR::LDS_E;  // LDS elements per 16-byte fragment chunk
     using SA = Stager<AR, AKM, BM, BK, NT>;
     using SB = Stager<AR, BKM, BN, BK, NT>;
     const bool AKC = AKM == 2 ? (j.akc != 0) : (AKM == 1);
@@ -401,30 +432,7 @@ __device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
     };
     auto compute = [&](int buf) {
         const char* la = smem + buf * S::BUF;
-        const char* lb = la + AR::NPLANE * S::A_PLANE;
-#pragma unroll
-        for (int ks = 0; ks < BK / AR::KSTEP; ++ks) {
-            const int kbyte = (ks * AR::KSTEP + (lane >> 4) * KC_L) * AR::LDS_E;
-            u32x4 af[S::FM][AR::NPLANE], bfg[S::FN][AR::NPLANE];
-#pragma unroll
-            for (int fm = 0; fm < S::FM; ++fm) {
-                const int row = wm * S::WTM + fm * 16 + frag_row<AR>(lane & 15);
-#pragma unroll
-                for (int pl = 0; pl < AR::NPLANE; ++pl)
-                    af[fm][pl] = *reinterpret_cast<const u32x4*>(la + pl * S::A_PLANE + row * S::ROWB + kbyte);
-            }
-#pragma unroll
-            for (int fn = 0; fn < S::FN; ++fn) {
-                const int row = wn * S::WTN + fn * 16 + (lane & 15);
-#pragma unroll
-                for (int pl = 0; pl < AR::NPLANE; ++pl)
-                    bfg[fn][pl] = *reinterpret_cast<const u32x4*>(lb + pl * S::B_PLANE + row * S::ROWB + kbyte);
-            }
-#pragma unroll
-            for (int fm = 0; fm < S::FM; ++fm)
-#pragma unroll
-                for (int fn = 0; fn < S::FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
-        }
+        tile_compute<AR, S, BK>(acc, la, la + AR::NPLANE * S::A_PLANE, wm, wn, lane);
     };
 
     if constexpr (STAGES == 1) {

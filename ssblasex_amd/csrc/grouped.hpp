@@ -1,5 +1,6 @@
-// grouped.hpp -- device data of a grouped plan (shared by grouped.hip and the
-// host plan builder).  Plain structs, no device code.
+// grouped.hpp -- device data of a grouped plan (shared by the grouped kernel
+// translation units and the host plan builder).  Plain structs, no device
+// code.
 #pragma once
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
@@ -8,8 +9,21 @@
 
 namespace ssb {
 
-// Tile shapes of the grouped kernel (all 4 waves, LDS double-buffered).
+// Tile shapes of the grouped kernel (4 waves in a 2x2 grid, LDS
+// double-buffered, operand layouts read per item at run time).
 enum GroupedCfg : int { GCFG_32 = 0, GCFG_64 = 1, GCFG_128 = 2, GCFG_COUNT = 3 };
+
+// Kernel families: the arithmetic forms one grouped kernel can run.  int8
+// shares a kernel with whichever fp32 form the plan uses, so the usual
+// mixed int8/fp32 plan is a single launch.  fp64 groups run in a kernel of
+// their own.
+enum GroupedFamily : int {
+    GF_I8_F32 = 0,
+    GF_I8_BF16X3 = 1,
+    GF_I8_BF16X6 = 2,
+    GF_F64 = 3,
+    GF_COUNT = 4
+};
 
 struct GScalars {  // same layout as the device Scalars
     float af, bf;
@@ -19,7 +33,8 @@ struct GScalars {  // same layout as the device Scalars
 
 struct GGroup {
     int form;   // ArithKind
-    int ta, tb;
+    int akc;    // op(A) stored K-contiguous (A transposed)
+    int bkc;    // op(B) stored K-contiguous (B not transposed)
     int beta0;
     GScalars s;
 };
@@ -27,25 +42,57 @@ struct GGroup {
 struct GInst {
     int m, n, k, group;
     long lda, ldb, ldc;
+    uint32_t abytes, bbytes;  // extent of the stored A / B in bytes
 };
 
+// One output tile of one instance.
 struct GItem {
     int inst;
     int i0, j0;
-    int cfg;
+    int cfg;    // GroupedCfg
 };
 
 struct GroupedDevice {
-    const GItem* items;
+    const GItem* items;  // sorted by estimated cost, longest first
     int nitems;
     const GInst* insts;
     const GGroup* groups;
-    const void* const* A;
+    const void* const* A;  // the caller's device arrays of device pointers
     const void* const* B;
     void* const* C;
+    // two counters, zero between launches: [0] next item, [1] workgroups done.
+    // The last workgroup of a launch resets both, so launches of one plan
+    // must be ordered (one stream, or a graph replayed on one stream).
+    unsigned* counters;
 };
 
-hipError_t launch_grouped(const GroupedDevice& d, hipStream_t st);
-int grouped_lds_bytes();
+// k-depth of a grouped tile per form and shape.
+constexpr int grouped_bk(int form, int cfg)
+{
+    return form == K_I8 ? 64 : form == K_F64 ? 16 : (form == K_F32 && cfg == GCFG_128) ? 16 : 32;
+}
+// Largest tile shape per form: the bf16 split forms keep two or three
+// operand planes in LDS and fp64 holds twice the accumulator registers, so
+// their 128x128 tile would hold the family's kernel to one workgroup per CU.
+constexpr int grouped_max_cfg(int form)
+{
+    return (form == K_BF16X3 || form == K_BF16X6 || form == K_F64) ? GCFG_64 : GCFG_128;
+}
+
+// Names of the grouped tile shapes in the tuning table ("g32", "g64", "g128").
+const char* grouped_cfg_name(int cfg);
+// Work-class column value that marks grouped rows in the tuning table.
+constexpr int GROUPED_WORK_CLASS = 9;
+// Size class of one dimension for grouped tuning: <= 32, <= 64, <= 128, more.
+int grouped_size_class(long x);
+// Tuned grouped tile shape for an instance of the form, -1 when none.
+int grouped_tuned_config(int form, long m, long n, long k);
+
+// Enqueue one persistent launch of the family's kernel over d's items.
+hipError_t launch_grouped(int family, const GroupedDevice& d, hipStream_t st);
+hipError_t launch_grouped_i8_f32(const GroupedDevice& d, hipStream_t st);
+hipError_t launch_grouped_i8_bf16x3(const GroupedDevice& d, hipStream_t st);
+hipError_t launch_grouped_i8_bf16x6(const GroupedDevice& d, hipStream_t st);
+hipError_t launch_grouped_f64(const GroupedDevice& d, hipStream_t st);
 
 }  // namespace ssb

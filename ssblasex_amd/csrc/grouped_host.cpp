@@ -1,28 +1,386 @@
-// grouped_host.cpp -- grouped entry points (placeholder until the grouped
-// kernel lands: every call reports NOTIMPLEMENTED_ERROR).
+// grouped_host.cpp -- grouped plans: argument checks, the tile shape of each
+// instance, the cost-sorted device work list, execution and graph capture.
+#include <limits.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <new>
+#include <vector>
+
 #include "api_internal.hpp"
+#include "grouped.hpp"
+
+struct ssblasGroupedPlan {
+    int device = 0;
+    void* mem = nullptr;  // counters, groups, instances, items: one device allocation
+    int nfam = 0;
+    int fam_id[ssb::GF_COUNT] = {};
+    ssb::GroupedDevice fam[ssb::GF_COUNT] = {};
+    std::mutex mu;  // guards the lazily captured graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace ssb {
+
+const char* grouped_cfg_name(int cfg)
+{
+    static const char* names[GCFG_COUNT] = {"g32", "g64", "g128"};
+    return cfg >= 0 && cfg < GCFG_COUNT ? names[cfg] : "?";
+}
+
+hipError_t launch_grouped(int family, const GroupedDevice& d, hipStream_t st)
+{
+    switch (family) {
+    case GF_I8_F32: return launch_grouped_i8_f32(d, st);
+    case GF_I8_BF16X3: return launch_grouped_i8_bf16x3(d, st);
+    case GF_I8_BF16X6: return launch_grouped_i8_bf16x6(d, st);
+    default: return launch_grouped_f64(d, st);
+    }
+}
+
+namespace {
+
+int tile_of(int cfg) { return cfg == GCFG_32 ? 32 : cfg == GCFG_64 ? 64 : 128; }
+
+int elem_bytes_in(int form) { return form == K_I8 ? 1 : form == K_F64 ? 8 : 4; }
+int elem_bytes_out(int form) { return form == K_F64 ? 8 : 4; }
+
+// Estimated time (ns) of one item: the k-blocks at the slower of MFMA and
+// memory rate for one of the two workgroups a CU holds, a floor per k-block
+// for the load latency the double buffer does not cover, and a fixed cost
+// for the first load and the epilogue.
+double item_ns(int form, int cfg, long k, bool beta0)
+{
+    // per-CU dense MFMA rate in ops/ns at 2.4 GHz, split forms at their product count
+    static const double rate[5] = {8192 * 2.4, 256 * 2.4, 128 * 2.4, 4096 * 2.4 / 3, 4096 * 2.4 / 6};
+    const double cu_bw = 8000.0 / 256.0;  // bytes/ns per CU with every CU busy
+    const int t = tile_of(cfg), bk = grouped_bk(form, cfg);
+    const long nk = (k + bk - 1) / bk;
+    const double mma = 2.0 * t * t * bk / (rate[form] / 2);
+    const double mem = 2.0 * t * bk * elem_bytes_in(form) / (cu_bw / 2);
+    const double epi = (double)t * t * elem_bytes_out(form) * (beta0 ? 1 : 2) / (cu_bw / 2);
+    return nk * std::max(std::max(mma, mem), 250.0) + epi + 1000.0;
+}
+
+long tiles(int cfg, long m, long n)
+{
+    const long t = tile_of(cfg);
+    return ((m + t - 1) / t) * ((n + t - 1) / t);
+}
+
+int pick_cfg(int form, long m, long n, long k, bool beta0)
+{
+    const char* forced = getenv("SSBLAS_GROUPED_CFG");
+    int top = grouped_max_cfg(form);
+    if (forced) {
+        for (int c = 0; c < GCFG_COUNT; ++c)
+            if (strcmp(forced, grouped_cfg_name(c)) == 0) return std::min(c, top);
+    }
+    int tuned = grouped_tuned_config(form, m, n, k);
+    if (tuned >= 0) return std::min(tuned, top);
+    int best = GCFG_32;
+    double best_ns = 1e300;
+    for (int c = GCFG_32; c <= top; ++c) {
+        double ns = tiles(c, m, n) * item_ns(form, c, k, beta0);
+        if (ns < best_ns) {
+            best_ns = ns;
+            best = c;
+        }
+    }
+    return best;
+}
+
+long stored_rows(ssblasOperation_t op, long rows_of_op, long cols_of_op)
+{
+    return op == SSBLAS_OP_N ? rows_of_op : cols_of_op;
+}
+
+// Bytes a buffer descriptor for the stored matrix spans; 0 when it (with the
+// read-ahead of one tile past its edge) does not fit 31-bit offsets.
+uint32_t extent_bytes(long rows, long cols, long ld, int e)
+{
+    const long ext = ((cols - 1) * ld + rows) * (long)e;
+    const long reach = ext + 256L * (ld + 1) * e;
+    if (reach >= (1L << 31)) return 0;
+    return (uint32_t)ext;
+}
+
+struct HostItem {
+    GItem it;
+    double ns;
+};
+
+}  // namespace
+
+ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const SSINT group_size[],
+                              const ssblasOperation_t transa_array[], const ssblasOperation_t transb_array[],
+                              const ssblasDataType_t Atype_array[], const ssblasDataType_t Btype_array[],
+                              const ssblasDataType_t Ctype_array[], const void* const alpha_array[],
+                              const void* const beta_array[], const SSINT m_array[], const SSINT n_array[],
+                              const SSINT k_array[], const void* const A[], const SSINT lda_array[],
+                              const void* const B[], const SSINT ldb_array[], void* const C[],
+                              const SSINT ldc_array[], ssblasComputeType_t computeType)
+{
+    if (plan == nullptr) return SSBLAS_STATUS_INVALID_VALUE;
+    *plan = nullptr;
+    if (group_count < 0) return SSBLAS_STATUS_INVALID_VALUE;
+    if (group_count == 0) return SSBLAS_STATUS_INTERNAL_ERROR;  // zero tasks, as batchCount == 0
+    if (!group_size || !transa_array || !transb_array || !Atype_array || !Btype_array || !Ctype_array ||
+        !alpha_array || !beta_array)
+        return SSBLAS_STATUS_INVALID_VALUE;
+    if (group_count > INT_MAX) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
+
+    // computeType: applies to the fp32 groups (plain or a bf16 form) and to
+    // fp64 groups (plain only); int8 groups ignore it.
+    const int ct = (int)computeType;
+    int fp32_form = K_F32;
+    if (ct == SSBLAS_COMPUTE_32F_FAST_BF16X3) fp32_form = K_BF16X3;
+    else if (ct == SSBLAS_COMPUTE_32F_EMULATED_BF16X6) fp32_form = K_BF16X6;
+    else if (ct == SSBLAS_COMPUTE_64F_EMULATED_OZAKI ||
+             (ct >= SSBLAS_COMPUTE_64F_EMULATED_OZAKI_S3 && ct <= SSBLAS_COMPUTE_64F_EMULATED_OZAKI_S9))
+        return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;  // grouped fp64 runs the plain form only
+    else if (ct != SSBLAS_COMPUTE_DEFAULT_TYPE && ct != SSBLAS_COMPUTE_32I && ct != SSBLAS_COMPUTE_32F &&
+             ct != SSBLAS_COMPUTE_64F)
+        return SSBLAS_STATUS_INVALID_VALUE;
+
+    long total = 0;
+    for (long g = 0; g < group_count; ++g) {
+        if (group_size[g] < 0) return SSBLAS_STATUS_INVALID_VALUE;
+        total += group_size[g];
+        if (total > INT_MAX) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
+    }
+    if (total == 0) return SSBLAS_STATUS_INTERNAL_ERROR;
+    if (!m_array || !n_array || !k_array || !lda_array || !ldb_array || !ldc_array) return SSBLAS_STATUS_INVALID_VALUE;
+
+    std::vector<GGroup> groups((size_t)group_count);
+    for (long g = 0; g < group_count; ++g) {
+        TypeTriple tt = classify_types(Atype_array[g], Btype_array[g], Ctype_array[g]);
+        if (tt == TT_NONE) return SSBLAS_STATUS_INVALID_VALUE;
+        const ssblasOperation_t ta = transa_array[g], tb = transb_array[g];
+        if ((ta != SSBLAS_OP_N && ta != SSBLAS_OP_T) || (tb != SSBLAS_OP_N && tb != SSBLAS_OP_T))
+            return SSBLAS_STATUS_INVALID_VALUE;
+        if (alpha_array[g] == nullptr || beta_array[g] == nullptr) return SSBLAS_STATUS_INVALID_VALUE;
+        GemmProblem sc;
+        fill_scalars(tt, alpha_array[g], beta_array[g], sc);
+        GGroup& G = groups[(size_t)g];
+        G.form = tt == TT_I8 ? K_I8 : tt == TT_F64 ? K_F64 : fp32_form;
+        G.akc = ta == SSBLAS_OP_T ? 1 : 0;
+        G.bkc = tb == SSBLAS_OP_N ? 1 : 0;
+        G.beta0 = sc.beta0 ? 1 : 0;
+        G.s = GScalars{sc.af, sc.bf, sc.ad, sc.bd, sc.ai, sc.bi};
+    }
+
+    std::vector<GInst> insts((size_t)total);
+    long idx = 0;
+    for (long g = 0; g < group_count; ++g) {
+        const ssblasOperation_t ta = transa_array[g], tb = transb_array[g];
+        const int form = groups[(size_t)g].form;
+        const int e = elem_bytes_in(form);
+        for (long i = 0; i < group_size[g]; ++i, ++idx) {
+            const long m = m_array[idx], n = n_array[idx], k = k_array[idx];
+            const long lda = lda_array[idx], ldb = ldb_array[idx], ldc = ldc_array[idx];
+            if (m < 0 || n < 0 || k < 0) return SSBLAS_STATUS_INVALID_VALUE;
+            const long ra = stored_rows(ta, m, k), ca = stored_rows(ta, k, m);
+            const long rb = stored_rows(tb, k, n), cb = stored_rows(tb, n, k);
+            if (lda < std::max(ra, 1L) || ldb < std::max(rb, 1L) || ldc < std::max(m, 1L))
+                return SSBLAS_STATUS_INVALID_VALUE;
+            if (m > INT_MAX || n > INT_MAX || k > INT_MAX) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
+            GInst& I = insts[(size_t)idx];
+            I.m = (int)m;
+            I.n = (int)n;
+            I.k = (int)k;
+            I.group = (int)g;
+            I.lda = lda;
+            I.ldb = ldb;
+            I.ldc = ldc;
+            I.abytes = I.bbytes = 0;
+            if (m == 0 || n == 0 || k == 0) continue;  // nothing to compute (k == 0: C unchanged)
+            I.abytes = extent_bytes(ra, ca, lda, e);
+            I.bbytes = extent_bytes(rb, cb, ldb, e);
+            if (I.abytes == 0 || I.bbytes == 0) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
+        }
+    }
+    if (A == nullptr || B == nullptr || C == nullptr) return SSBLAS_STATUS_INVALID_VALUE;
+    if (!device_accessible(A) || !device_accessible(B) || !device_accessible(C))
+        return SSBLAS_STATUS_INVALID_VALUE;
+
+    // Work list per family, longest items first.
+    const int other_family = fp32_form == K_BF16X3 ? GF_I8_BF16X3 : fp32_form == K_BF16X6 ? GF_I8_BF16X6 : GF_I8_F32;
+    std::vector<HostItem> lists[GF_COUNT];
+    for (long i = 0; i < total; ++i) {
+        const GInst& I = insts[(size_t)i];
+        if (I.m == 0 || I.n == 0 || I.k == 0) continue;
+        const GGroup& G = groups[(size_t)I.group];
+        const int cfg = pick_cfg(G.form, I.m, I.n, I.k, G.beta0 != 0);
+        const int t = tile_of(cfg);
+        const double ns = item_ns(G.form, cfg, I.k, G.beta0 != 0);
+        auto& L = lists[G.form == K_F64 ? GF_F64 : other_family];
+        for (int j0 = 0; j0 < I.n; j0 += t)
+            for (int i0 = 0; i0 < I.m; i0 += t) L.push_back(HostItem{GItem{(int)i, i0, j0, cfg}, ns});
+    }
+
+    long nitems = 0;
+    for (auto& L : lists) {
+        std::stable_sort(L.begin(), L.end(), [](const HostItem& a, const HostItem& b) { return a.ns > b.ns; });
+        nitems += (long)L.size();
+    }
+    if (nitems > INT_MAX) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
+
+    // Device image: [counters][groups][instances][items of each family]
+    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t off_groups = 256;
+    const size_t off_insts = align(off_groups + groups.size() * sizeof(GGroup));
+    const size_t off_items = align(off_insts + insts.size() * sizeof(GInst));
+    const size_t bytes = off_items + (size_t)std::max(nitems, 1L) * sizeof(GItem);
+    std::vector<char> image(bytes, 0);
+    memcpy(image.data() + off_groups, groups.data(), groups.size() * sizeof(GGroup));
+    memcpy(image.data() + off_insts, insts.data(), insts.size() * sizeof(GInst));
+
+    ssblasGroupedPlan* p = new (std::nothrow) ssblasGroupedPlan();
+    if (p == nullptr) return SSBLAS_STATUS_ALLOC_FAILED;
+    hipError_t e = hipGetDevice(&p->device);
+    if (e == hipSuccess) e = hipMalloc(&p->mem, bytes);
+    if (e != hipSuccess) {
+        delete p;
+        return hip_status(e);
+    }
+    char* dev = static_cast<char*>(p->mem);
+    size_t pos = off_items;
+    for (int f = 0; f < GF_COUNT; ++f) {
+        auto& L = lists[f];
+        if (L.empty()) continue;
+        GItem* dst = reinterpret_cast<GItem*>(image.data() + pos);
+        for (size_t i = 0; i < L.size(); ++i) dst[i] = L[i].it;
+        GroupedDevice& D = p->fam[p->nfam];
+        D.items = reinterpret_cast<const GItem*>(dev + pos);
+        D.nitems = (int)L.size();
+        D.insts = reinterpret_cast<const GInst*>(dev + off_insts);
+        D.groups = reinterpret_cast<const GGroup*>(dev + off_groups);
+        D.A = A;
+        D.B = B;
+        D.C = C;
+        D.counters = reinterpret_cast<unsigned*>(dev) + 2 * f;
+        p->fam_id[p->nfam] = f;
+        ++p->nfam;
+        pos += L.size() * sizeof(GItem);
+    }
+    e = hipMemcpy(p->mem, image.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(p->mem);
+        delete p;
+        return hip_status(e);
+    }
+    *plan = p;
+    return SSBLAS_STATUS_SUCCESS;
+}
+
+ssblasStatus_t grouped_execute(ssblasGroupedPlan_t p, hipStream_t stream)
+{
+    if (p == nullptr) return SSBLAS_STATUS_INVALID_VALUE;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev != p->device) return SSBLAS_STATUS_INVALID_VALUE;
+    for (int i = 0; i < p->nfam; ++i) {
+        hipError_t e = launch_grouped(p->fam_id[i], p->fam[i], stream);
+        if (e != hipSuccess) return hip_status(e);
+    }
+    return SSBLAS_STATUS_SUCCESS;
+}
+
+ssblasStatus_t grouped_graph_launch(ssblasGroupedPlan_t p, hipStream_t stream)
+{
+    if (p == nullptr) return SSBLAS_STATUS_INVALID_VALUE;
+    {
+        std::lock_guard<std::mutex> lock(p->mu);
+        if (p->exec == nullptr) {
+            hipStream_t cs = nullptr;
+            hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+            if (e != hipSuccess) return hip_status(e);
+            e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+            ssblasStatus_t st = SSBLAS_STATUS_SUCCESS;
+            if (e == hipSuccess) {
+                st = grouped_execute(p, cs);
+                hipGraph_t g = nullptr;
+                hipError_t e2 = hipStreamEndCapture(cs, &g);
+                if (e2 == hipSuccess) p->graph = g;
+                if (e == hipSuccess) e = e2;
+            }
+            if (e == hipSuccess && st == SSBLAS_STATUS_SUCCESS)
+                e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+            (void)hipStreamDestroy(cs);
+            if (st != SSBLAS_STATUS_SUCCESS) return st;
+            if (e != hipSuccess) return hip_status(e);
+        }
+    }
+    return hip_status(hipGraphLaunch(p->exec, stream));
+}
+
+ssblasStatus_t grouped_destroy(ssblasGroupedPlan_t p)
+{
+    if (p == nullptr) return SSBLAS_STATUS_SUCCESS;
+    // work of this plan may still be queued on any stream
+    (void)hipDeviceSynchronize();
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->mem) (void)hipFree(p->mem);
+    delete p;
+    return SSBLAS_STATUS_SUCCESS;
+}
+
+}  // namespace ssb
 
 extern "C" {
-ssblasStatus_t ssblasGroupedPlanCreate(ssblasGroupedPlan_t* plan, SSINT, const SSINT[], const ssblasOperation_t[],
-                                       const ssblasOperation_t[], const ssblasDataType_t[],
-                                       const ssblasDataType_t[], const ssblasDataType_t[], const void* const[],
-                                       const void* const[], const SSINT[], const SSINT[], const SSINT[],
-                                       const void* const[], const SSINT[], const void* const[], const SSINT[],
-                                       void* const[], const SSINT[], ssblasComputeType_t, ssblasGemmAlgo_t)
+
+ssblasStatus_t ssblasGroupedPlanCreate(ssblasGroupedPlan_t* plan, SSINT group_count, const SSINT group_size[],
+                                       const ssblasOperation_t transa_array[],
+                                       const ssblasOperation_t transb_array[], const ssblasDataType_t Atype_array[],
+                                       const ssblasDataType_t Btype_array[], const ssblasDataType_t Ctype_array[],
+                                       const void* const alpha_array[], const void* const beta_array[],
+                                       const SSINT m_array[], const SSINT n_array[], const SSINT k_array[],
+                                       const void* const A[], const SSINT lda_array[], const void* const B[],
+                                       const SSINT ldb_array[], void* const C[], const SSINT ldc_array[],
+                                       ssblasComputeType_t computeType, ssblasGemmAlgo_t)
 {
-    if (plan) *plan = nullptr;
-    return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
+    return ssb::grouped_create(plan, group_count, group_size, transa_array, transb_array, Atype_array, Btype_array,
+                               Ctype_array, alpha_array, beta_array, m_array, n_array, k_array, A, lda_array, B,
+                               ldb_array, C, ldc_array, computeType);
 }
-ssblasStatus_t ssblasGroupedPlanExecute(ssblasGroupedPlan_t, ssblasStream_t) { return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR; }
-ssblasStatus_t ssblasGroupedPlanGraphLaunch(ssblasGroupedPlan_t, ssblasStream_t) { return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR; }
-ssblasStatus_t ssblasGroupedPlanDestroy(ssblasGroupedPlan_t) { return SSBLAS_STATUS_SUCCESS; }
-ssblasStatus_t ssblasGemmGroupedBatchedEx(SSINT, const SSINT[], const ssblasOperation_t[], const ssblasOperation_t[],
-                                          const ssblasDataType_t[], const ssblasDataType_t[],
-                                          const ssblasDataType_t[], const void* const[], const void* const[],
-                                          const SSINT[], const SSINT[], const SSINT[], const void* const[],
-                                          const SSINT[], const void* const[], const SSINT[], void* const[],
-                                          const SSINT[], ssblasComputeType_t, ssblasGemmAlgo_t)
+
+ssblasStatus_t ssblasGroupedPlanExecute(ssblasGroupedPlan_t plan, ssblasStream_t stream)
 {
-    return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
+    return ssb::grouped_execute(plan, reinterpret_cast<hipStream_t>(stream));
 }
+
+ssblasStatus_t ssblasGroupedPlanGraphLaunch(ssblasGroupedPlan_t plan, ssblasStream_t stream)
+{
+    return ssb::grouped_graph_launch(plan, reinterpret_cast<hipStream_t>(stream));
 }
+
+ssblasStatus_t ssblasGroupedPlanDestroy(ssblasGroupedPlan_t plan) { return ssb::grouped_destroy(plan); }
+
+ssblasStatus_t ssblasGemmGroupedBatchedEx(SSINT group_count, const SSINT group_size[],
+                                          const ssblasOperation_t transa_array[],
+                                          const ssblasOperation_t transb_array[],
+                                          const ssblasDataType_t Atype_array[], const ssblasDataType_t Btype_array[],
+                                          const ssblasDataType_t Ctype_array[], const void* const alpha_array[],
+                                          const void* const beta_array[], const SSINT m_array[],
+                                          const SSINT n_array[], const SSINT k_array[], const void* const A[],
+                                          const SSINT lda_array[], const void* const B[], const SSINT ldb_array[],
+                                          void* const C[], const SSINT ldc_array[], ssblasComputeType_t computeType,
+                                          ssblasGemmAlgo_t algo)
+{
+    ssblasGroupedPlan_t plan = nullptr;
+    ssblasStatus_t st = ssblasGroupedPlanCreate(&plan, group_count, group_size, transa_array, transb_array,
+                                                Atype_array, Btype_array, Ctype_array, alpha_array, beta_array,
+                                                m_array, n_array, k_array, A, lda_array, B, ldb_array, C, ldc_array,
+                                                computeType, algo);
+    if (st != SSBLAS_STATUS_SUCCESS) return st;
+    st = ssb::grouped_execute(plan, nullptr);
+    if (st == SSBLAS_STATUS_SUCCESS) st = ssb::hip_status(hipStreamSynchronize(nullptr));
+    ssb::grouped_destroy(plan);
+    return st;
+}
+
+}  // extern "C"

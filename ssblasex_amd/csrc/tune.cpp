@@ -2,7 +2,9 @@
 // `python -m ssblasex_amd.tune` and read here at the first call.
 //
 // Table lines: "form ta tb mb nb kb wb config us" (see tune.py for the bucket
-// definitions).  Path: $SSBLAS_TUNE_TABLE, else tune/<arch>.txt next to the
+// definitions).  Grouped-plan rows carry wb = GROUPED_WORK_CLASS, ta = tb = 0,
+// size classes (<= 32, <= 64, <= 128, larger) in mb/nb/kb and a grouped tile
+// name (g32, g64, g128).  Path: $SSBLAS_TUNE_TABLE, else tune/<arch>.txt next to the
 // lib/ directory that holds this library.
 #include <dlfcn.h>
 #include <math.h>
@@ -16,6 +18,7 @@
 #include <tuple>
 
 #include "api_internal.hpp"
+#include "grouped.hpp"
 
 namespace ssb {
 
@@ -71,6 +74,8 @@ void load_table()
         if (sscanf(line, "%d %d %d %d %d %d %d %63s", &form, &ta, &tb, &mb, &nb, &kb, &wb, name) != 8) continue;
         for (int c = 0; c < CFG_COUNT; ++c)
             if (strcmp(name, config_name(c)) == 0) g_table[Key(form, ta, tb, mb, nb, kb, wb)] = c;
+        for (int c = 0; c < GCFG_COUNT; ++c)
+            if (strcmp(name, grouped_cfg_name(c)) == 0) g_table[Key(form, ta, tb, mb, nb, kb, wb)] = c;
     }
     fclose(f);
 }
@@ -82,6 +87,17 @@ int tuned_config(const GemmProblem& g)
     if (g_table.empty()) return -1;
     auto it = g_table.find(Key(g.arith, g.ta ? 1 : 0, g.tb ? 1 : 0, bucket(g.m), bucket(g.n), bucket(g.k),
                                work_class(g.m, g.n, g.batch)));
+    return it == g_table.end() ? -1 : it->second;
+}
+
+int grouped_size_class(long x) { return x <= 32 ? 0 : x <= 64 ? 1 : x <= 128 ? 2 : 3; }
+
+int grouped_tuned_config(int form, long m, long n, long k)
+{
+    std::call_once(g_once, load_table);
+    if (g_table.empty()) return -1;
+    auto it = g_table.find(Key(form, 0, 0, grouped_size_class(m), grouped_size_class(n), grouped_size_class(k),
+                               GROUPED_WORK_CLASS));
     return it == g_table.end() ? -1 : it->second;
 }
 
