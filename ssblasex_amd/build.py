@@ -6,7 +6,9 @@
 HIP translation units (*.hip) are compiled with hipcc for gfx950, host-only
 units (*.cpp) with g++ against the HIP runtime headers, and everything is
 linked into ssblasex_amd/lib/libssblasGemmBatchedEx.so.  Objects go to
-ssblasex_amd/_build/.
+ssblasex_amd/_build/.  The example-compatible driver
+(ssblasex_amd/example/ssblas_example.cpp) is linked against it into
+ssblasex_amd/lib/ssblas_example.
 """
 from __future__ import annotations
 
@@ -90,7 +92,26 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -
         if r.returncode != 0:
             raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}{r.stderr}")
         os.replace(tmp, lib)
+    _build_example(lib, force)
     return lib
+
+
+def example_path() -> Path:
+    return LIBDIR / "ssblas_example"
+
+
+def _build_example(lib: Path, force: bool) -> None:
+    """The example-compatible command-line driver, linked against the library."""
+    src = PKG / "example" / "ssblas_example.cpp"
+    exe = example_path()
+    if not force and not _needs(exe, [src, lib] + _headers()):
+        return
+    cmd = [os.environ.get("CXX", "g++"), *CXX_FLAGS, "-fopenmp", str(src), "-o", str(exe),
+           f"-L{LIBDIR}", "-lssblasGemmBatchedEx", f"-L{ROCM / 'lib'}", "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM / 'lib'}"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"example build failed\n{' '.join(cmd)}\n{r.stdout}{r.stderr}")
 
 
 def main() -> None:

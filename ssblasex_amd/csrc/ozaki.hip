@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+#include <stdlib.h>
 
 #include "gemm_kernels.hpp"
 #include "launch.hpp"
@@ -306,6 +307,205 @@ __global__ void __launch_bounds__(256) ozaki_gemm(const OzParams p)
 }
 
 // ---------------------------------------------------------------------------
+// fused form for small instances (m, n, k <= 32): one wave per instance,
+// nothing in HBM but A, B and C.
+//
+// Slicing: lane l owns row l % 32 of op(A) and column l % 32 of op(B) over
+// the k half kh = l / 32 (16 doubles each); the row / column maximum joins
+// the two halves with one lane exchange.  Each slice of 16 values is one
+// 16-byte chunk, written to the wave's LDS area
+//     [operand][slice p][row 0..31][2 chunks of 16 k]
+// with the chunk index XOR-ed with bit 2 of the row, so the 16-lane
+// fragment reads below touch distinct banks.
+//
+// Products: one v_mfma_i32_16x16x64_i8 consumes k = 64: lane groups 0, 1
+// (lanes 0..31) carry the 32 k of slice pair (p, t-p) and groups 2, 3 those
+// of pair (p+1, t-p-1), so each MFMA adds two products of the same order t,
+// whose sum is what D_t needs.  D_t is exact in int32 (2 * 9 * 32 * 127^2 <
+// 2^31) and the fold, exponents and epilogue are ozaki_gemm's, so the
+// result is bit-identical to the general path.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int oz_chunk_off(int p, int row, int half)
+{
+    return p * 1024 + row * 32 + 16 * (half ^ ((row >> 2) & 1));
+}
+
+template <int S>
+__device__ __forceinline__ void oz_slice_to_lds(const double* x, int e, char* lds, int row, int half)
+{
+    double v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = (e == OZ_NONFINITE) ? 0.0 : ldexp(x[i], -e);
+#pragma unroll
+    for (int p = 0; p < S; ++p) {
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            v[i] *= 128.0;
+            const double t = trunc(v[i]);
+            v[i] -= t;
+            w[i >> 2] |= ((uint32_t)(int32_t)t & 0xffu) << (8 * (i & 3));
+        }
+        *reinterpret_cast<u32x4*>(lds + oz_chunk_off(p, row, half)) = u32x4{w[0], w[1], w[2], w[3]};
+    }
+}
+
+// Exponent of the max |.| over both k halves (lanes l and l ^ 32).
+__device__ __forceinline__ int oz_exp16(const double* x)
+{
+    double mx = 0.0;
+    bool bad = false;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const double a = fabs(x[i]);
+        bad |= !isfinite(a);
+        mx = fmax(mx, a);
+    }
+    mx = fmax(mx, __shfl_xor(mx, 32));
+    bad |= __shfl_xor((int)bad, 32) != 0;
+    return exp_of_max(mx, bad);
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) ozaki_small(const OzParams p)
+{
+    __shared__ __attribute__((aligned(16))) char smem[4][2 * S * 1024];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int inst = blockIdx.x * 4 + wave;
+    if (inst >= p.batch) return;
+    char* la = smem[wave];
+    char* lb = la + S * 1024;
+    const double* A = static_cast<const double*>(p.A[inst]);
+    const double* B = static_cast<const double*>(p.B[inst]);
+
+    // slices of row (lane % 32) of op(A) and column (lane % 32) of op(B), k half lane / 32
+    const int own = lane & 31, kh = lane >> 5;
+    int ea, eb;
+    {
+        double x[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int kk = kh * 16 + q;
+            x[q] = 0.0;
+            if (own < p.m && kk < p.k) x[q] = p.ta ? A[kk + (long)own * p.lda] : A[own + (long)kk * p.lda];
+        }
+        ea = oz_exp16(x);
+        oz_slice_to_lds<S>(x, ea, la, own, kh);
+    }
+    {
+        double x[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int kk = kh * 16 + q;
+            x[q] = 0.0;
+            if (own < p.n && kk < p.k) x[q] = p.tb ? B[own + (long)kk * p.ldb] : B[kk + (long)own * p.ldb];
+        }
+        eb = oz_exp16(x);
+        oz_slice_to_lds<S>(x, eb, lb, own, kh);
+    }
+    // the wave's own LDS writes are read back by other lanes of the wave
+    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+
+    const int r = lane & 15, g = lane >> 4, half = g & 1;
+    const bool hi = g >= 2;
+    f64x4 D[2][2];
+#pragma unroll
+    for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < 2; ++fn) D[fm][fn] = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int t = S + 1; t >= 2; --t) {
+        const int plo = t - S > 1 ? t - S : 1, phi = S < t - 1 ? S : t - 1;
+        i32x4 acc[2][2];
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < 2; ++fn) acc[fm][fn] = i32x4{0, 0, 0, 0};
+#pragma unroll
+        for (int pa = plo; pa <= phi; pa += 2) {
+            const int pb = pa + 1;  // second pair of the step, when pb <= phi
+            const bool two = pb <= phi;
+            // slice numbers (1-based) this lane feeds: pair a in groups 0, 1, pair b in 2, 3
+            const int sa_ = hi ? pb : pa, sb_ = t - sa_;
+            u32x4 af[2], bf[2];
+#pragma unroll
+            for (int f = 0; f < 2; ++f) {
+                const int row = f * 16 + r;
+                if (!hi || two) {
+                    af[f] = *reinterpret_cast<const u32x4*>(la + oz_chunk_off(sa_ - 1, row, half));
+                    bf[f] = *reinterpret_cast<const u32x4*>(lb + oz_chunk_off(sb_ - 1, row, half));
+                } else {
+                    af[f] = u32x4{0u, 0u, 0u, 0u};
+                    bf[f] = u32x4{0u, 0u, 0u, 0u};
+                }
+            }
+#pragma unroll
+            for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < 2; ++fn) ArithI8::mma(acc[fm][fn], &af[fm], &bf[fn]);
+        }
+        const double scale = ldexp(1.0, -7 * t);
+#pragma unroll
+        for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < 2; ++fn) {
+                int32_t a4[4];
+                __builtin_memcpy(a4, &acc[fm][fn], 16);
+                double d4[4];
+                __builtin_memcpy(d4, &D[fm][fn], 32);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) d4[v] = __builtin_fma((double)a4[v], scale, d4[v]);
+                __builtin_memcpy(&D[fm][fn], d4, 32);
+            }
+    }
+
+    // epilogue: C = fma(D * 2^(e_i + f_j), alpha, fl(beta * C))
+    double* C = static_cast<double*>(p.C[inst]);
+    const bool beta0 = p.beta0 != 0;
+#pragma unroll
+    for (int fm = 0; fm < 2; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < 2; ++fn) {
+            const int c = fn * 16 + r;
+            const int fj = __shfl(eb, c);  // lane c holds column c
+            double d4[4];
+            __builtin_memcpy(d4, &D[fm][fn], 32);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = fm * 16 + 4 * g + v;
+                const int ei = __shfl(ea, row);  // lane `row` holds that row
+                if (row < p.m && c < p.n) {
+                    const double val =
+                        (ei == OZ_NONFINITE || fj == OZ_NONFINITE) ? __builtin_nan("") : ldexp(d4[v], ei + fj);
+                    double* dst = C + (long)c * p.ldc + row;
+                    const double t0 = beta0 ? 0.0 : (*dst) * p.sc.bd;
+                    *dst = __builtin_fma(val, p.sc.ad, t0);
+                }
+            }
+        }
+}
+
+template <int S>
+static hipError_t run_ozaki_small(const OzParams& p, hipStream_t st)
+{
+    hipLaunchKernelGGL(ozaki_small<S>, dim3((unsigned)((p.batch + 3) / 4)), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+static hipError_t ozaki_small_dispatch(const OzParams& p, hipStream_t st)
+{
+    switch (p.s) {
+    case 3: return run_ozaki_small<3>(p, st);
+    case 4: return run_ozaki_small<4>(p, st);
+    case 5: return run_ozaki_small<5>(p, st);
+    case 6: return run_ozaki_small<6>(p, st);
+    case 7: return run_ozaki_small<7>(p, st);
+    case 8: return run_ozaki_small<8>(p, st);
+    default: return run_ozaki_small<9>(p, st);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
 template <bool AKC, bool BKC>
@@ -346,6 +546,27 @@ static hipError_t prep_operand(const void* const* X, long ld, int nrows, int nco
 hipError_t ozaki_run(const GemmProblem& g, int s, void* (*ws_get)(size_t, hipError_t*), hipStream_t st)
 {
     const int m = g.m, n = g.n, k = g.k;
+    if (m <= 32 && n <= 32 && k <= 32 && s >= 3 && s <= 9 && getenv("SSBLAS_OZAKI_NO_FUSED") == nullptr) {
+        // fused single-wave form: no workspace
+        OzParams p{};
+        p.A = g.A;
+        p.B = g.B;
+        p.C = g.C;
+        p.lda = g.lda;
+        p.ldb = g.ldb;
+        p.ldc = g.ldc;
+        p.m = m;
+        p.n = n;
+        p.k = k;
+        p.batch = g.batch;
+        p.s = s;
+        p.ta = g.ta;
+        p.tb = g.tb;
+        p.sc.ad = g.ad;
+        p.sc.bd = g.bd;
+        p.beta0 = g.beta0 ? 1 : 0;
+        return ozaki_small_dispatch(p, st);
+    }
     // slice storage keeps the operand's orientation, leading dimension
     // rounded up to 16 bytes
     const long ra = g.ta ? k : m, ca = g.ta ? m : k;  // storage of A

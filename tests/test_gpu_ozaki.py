@@ -28,7 +28,8 @@ def op_mats(b, i):
 
 @pytest.mark.parametrize("ta,tb", OPS)
 @pytest.mark.parametrize("s", [3, 5, 8])
-@pytest.mark.parametrize("shape", [(16, 16, 32), (37, 29, 70), (64, 64, 64), (100, 65, 129)], ids=str)
+@pytest.mark.parametrize("shape", [(1, 1, 1), (5, 32, 17), (16, 16, 32), (32, 32, 32), (37, 29, 70), (64, 64, 64),
+                                   (100, 65, 129)], ids=str)
 def test_ozaki_bit_exact_vs_model(ta, tb, s, shape):
     m, n, k = shape
     b = make_batch("f64", ta, tb, m, n, k, 2, alpha=2.0, beta=-0.5, pad=(1, 2, 3), seed=s * 7 + m, dist="wide")
@@ -95,3 +96,18 @@ def test_ozaki_config_fp64_32cubed_sample():
         res[s] = normwise(b, C)
     print("ozaki normwise error by slice count", res, "native", e_native, "reference order", e_ref)
     assert res[8] <= 2 * e_ref + UNIT_ROUNDOFF["f64"]
+
+
+@pytest.mark.parametrize("ta,tb", OPS)
+@pytest.mark.parametrize("s", [4, 9])
+def test_ozaki_fused_small_equals_general(ta, tb, s, monkeypatch):
+    """Instances with m, n, k <= 32 run the fused one-wave kernel; it must
+    give the general (workspace) path's result bit for bit."""
+    b = make_batch("f64", ta, tb, 32, 29, 31, 40, alpha=-1.5, beta=0.25, pad=(0, 3, 1), seed=11 + s, dist="wide")
+    st, fused = run_gpu(b, compute_type=0x100 + s)
+    assert st == 0
+    monkeypatch.setenv("SSBLAS_OZAKI_NO_FUSED", "1")
+    st, general = run_gpu(b, compute_type=0x100 + s)
+    assert st == 0
+    for x, y in zip(fused, general):
+        assert np.array_equal(x, y)
