@@ -41,6 +41,23 @@ enum ArithId : int {
 };
 
 // Scalars of one call, every representation filled by the host.
+// Global-address-space views of generic pointers.  Operand and result
+// pointers come out of pointer arrays, so the compiler cannot tell they are
+// global and would use FLAT instructions; a FLAT access counts against the
+// LDS wait counter too, so every wait for an LDS read would also wait for
+// the global loads and stores in flight.
+#define SSB_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const SSB_GLOBAL T* gbl(const T* p)
+{
+    return (const SSB_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ SSB_GLOBAL T* gbl(T* p)
+{
+    return (SSB_GLOBAL T*)p;
+}
+
 struct Scalars {
     float af, bf;
     double ad, bd;

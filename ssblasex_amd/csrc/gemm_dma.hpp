@@ -186,13 +186,13 @@ struct DmaTile {
             if (kc) { in_row = row < mnv; whole = in_row && 16 * c + 16 <= kv; }
             else { in_row = row < kv; whole = in_row && 16 * c + 16 <= mnv; }
             if (whole && (reinterpret_cast<uintptr_t>(X + off0) & 15) == 0) {
-                u32x4 v = *reinterpret_cast<const u32x4*>(X + off0);
+                u32x4 v = *gbl(reinterpret_cast<const u32x4*>(X + off0));
                 w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
             } else if (in_row) {
 #pragma unroll
                 for (int e = 0; e < 16; ++e) {
                     const bool ok = kc ? (16 * c + e < kv) : (16 * c + e < mnv);
-                    const uint32_t b = ok ? (uint32_t)(uint8_t)X[off0 + e] : 0u;
+                    const uint32_t b = ok ? (uint32_t)*gbl(reinterpret_cast<const uint8_t*>(X + off0 + e)) : 0u;
                     w[e >> 2] |= b << (8 * (e & 3));
                 }
             }

@@ -49,7 +49,7 @@ template <> struct Raw<8> { using T = uint64_t; };
 template <int E>
 __device__ __forceinline__ typename Raw<E>::T load_elem(const char* base, long off_elems)
 {
-    return *reinterpret_cast<const typename Raw<E>::T*>(base + off_elems * E);
+    return *gbl(reinterpret_cast<const typename Raw<E>::T*>(base + off_elems * E));
 }
 
 // ---------------------------------------------------------------------------
@@ -98,13 +98,13 @@ struct StagerImpl {
     __device__ __forceinline__ static void put_word(uint32_t* raw, int x, int w, const char* src)
     {
         if constexpr (WB == 16) {
-            u32x4 v = *reinterpret_cast<const u32x4*>(src);
+            u32x4 v = *gbl(reinterpret_cast<const u32x4*>(src));
             raw[x * UD + w * 4 + 0] = v.x;
             raw[x * UD + w * 4 + 1] = v.y;
             raw[x * UD + w * 4 + 2] = v.z;
             raw[x * UD + w * 4 + 3] = v.w;
         } else {
-            raw[x * UD + w] = *reinterpret_cast<const uint32_t*>(src);
+            raw[x * UD + w] = *gbl(reinterpret_cast<const uint32_t*>(src));
         }
     }
 
@@ -269,7 +269,7 @@ __device__ __forceinline__ void store_frag(typename AR::Out* C, long ldc, int r0
             // loop into one reused dword.
             Out old[4] = {Out(0), Out(0), Out(0), Out(0)};
             if (!beta0) {
-                u32x4 t = *reinterpret_cast<const u32x4*>(dst);
+                u32x4 t = *gbl(reinterpret_cast<const u32x4*>(dst));
                 uint32_t w[4] = {t.x, t.y, t.z, t.w};
                 __builtin_memcpy(old, w, 16);
             }
@@ -278,16 +278,16 @@ __device__ __forceinline__ void store_frag(typename AR::Out* C, long ldc, int r0
             for (int v = 0; v < 4; ++v) res[v] = epi(acc[v], old[v], sc, beta0);
             uint32_t w[4];
             __builtin_memcpy(w, res, 16);
-            *reinterpret_cast<u32x4*>(dst) = u32x4{w[0], w[1], w[2], w[3]};
+            *gbl(reinterpret_cast<u32x4*>(dst)) = u32x4{w[0], w[1], w[2], w[3]};
         } else {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 f64x2 old = {0.0, 0.0};
-                if (!beta0) old = *reinterpret_cast<const f64x2*>(dst + 2 * h);
+                if (!beta0) old = *gbl(reinterpret_cast<const f64x2*>(dst + 2 * h));
                 f64x2 res;
                 res[0] = epi((double)acc[2 * h], old[0], sc, beta0);
                 res[1] = epi((double)acc[2 * h + 1], old[1], sc, beta0);
-                *reinterpret_cast<f64x2*>(dst + 2 * h) = res;
+                *gbl(reinterpret_cast<f64x2*>(dst + 2 * h)) = res;
             }
         }
     } else {
@@ -297,8 +297,8 @@ __device__ __forceinline__ void store_frag(typename AR::Out* C, long ldc, int r0
             int r = r0 + v;
             if (r < m) {
                 Out* dst = C + (long)c * ldc + r;
-                Out o = beta0 ? Out(0) : *dst;
-                *dst = epi(acc[v], o, sc, beta0);
+                Out o = beta0 ? Out(0) : *gbl(dst);
+                *gbl(dst) = epi(acc[v], o, sc, beta0);
             }
         }
     }
@@ -541,7 +541,7 @@ __device__ __forceinline__ void direct_frag(const char* base, long sr, long sk, 
     if (KC && vec && r < rows && k0 + KL <= kvalid) {
 #pragma unroll
         for (int w = 0; w < NIN; ++w)
-            in[w] = *reinterpret_cast<const u32x4*>(base + ((long)r * sr + k0) * E + 16 * w);
+            in[w] = *gbl(reinterpret_cast<const u32x4*>(base + ((long)r * sr + k0) * E + 16 * w));
     } else {
         RT vals[KL];
 #pragma unroll

@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256) ozaki_exp_cols(const void* const* X, long
     double mx = 0.0;
     bool bad = false;
     for (int r = lane; r < nrows; r += 64) {
-        double v = fabs(x[r]);
+        double v = fabs(*gbl(x + r));
         bad |= !isfinite(v);
         mx = fmax(mx, v);
     }
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) ozaki_exp_rows(const void* const* X, long
     bool bad = false;
     if (r < nrows) {
         for (int c = wave; c < ncols; c += 4) {
-            double v = fabs(x[r + (long)c * ld]);
+            double v = fabs(*gbl(x + r + (long)c * ld));
             bad |= !isfinite(v);
             mx = fmax(mx, v);
         }
@@ -142,7 +142,7 @@ __global__ void __launch_bounds__(256) ozaki_slice(const void* const* X, long ld
         v[q] = 0.0;
         if (r < nrows) {
             const int ex = exp_by_col ? ei[c] : ei[r];
-            v[q] = (ex == OZ_NONFINITE) ? 0.0 : ldexp(x[r], -ex);
+            v[q] = (ex == OZ_NONFINITE) ? 0.0 : ldexp(*gbl(x + r), -ex);
         }
     }
     for (int p = 0; p < s; ++p) {
@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(256) ozaki_slice(const void* const* X, long ld
             v[q] -= t;
             word |= ((uint32_t)(int32_t)t & 0xffu) << (8 * q);
         }
-        *reinterpret_cast<uint32_t*>(out + p * plane) = word;
+        *gbl(reinterpret_cast<uint32_t*>(out + p * plane)) = word;
     }
 }
 
@@ -299,8 +299,8 @@ __global__ void __launch_bounds__(256) ozaki_gemm(const OzParams p)
                     const int ei = ea[r];
                     double val = (ei == OZ_NONFINITE || fj == OZ_NONFINITE) ? __builtin_nan("") : ldexp(d4[v], ei + fj);
                     double* dst = C + (long)c * p.ldc + r;
-                    double t0 = beta0 ? 0.0 : (*dst) * p.sc.bd;
-                    *dst = __builtin_fma(val, p.sc.ad, t0);
+                    double t0 = beta0 ? 0.0 : (*gbl(dst)) * p.sc.bd;
+                    *gbl(dst) = __builtin_fma(val, p.sc.ad, t0);
                 }
             }
     }
@@ -387,7 +387,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
         for (int q = 0; q < 16; ++q) {
             const int kk = kh * 16 + q;
             x[q] = 0.0;
-            if (own < p.m && kk < p.k) x[q] = p.ta ? A[kk + (long)own * p.lda] : A[own + (long)kk * p.lda];
+            if (own < p.m && kk < p.k) x[q] = *gbl(A + (p.ta ? kk + (long)own * p.lda : own + (long)kk * p.lda));
         }
         ea = oz_exp16(x);
         oz_slice_to_lds<S>(x, ea, la, own, kh);
@@ -398,7 +398,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
         for (int q = 0; q < 16; ++q) {
             const int kk = kh * 16 + q;
             x[q] = 0.0;
-            if (own < p.n && kk < p.k) x[q] = p.tb ? B[own + (long)kk * p.ldb] : B[kk + (long)own * p.ldb];
+            if (own < p.n && kk < p.k) x[q] = *gbl(B + (p.tb ? own + (long)kk * p.ldb : kk + (long)own * p.ldb));
         }
         eb = oz_exp16(x);
         oz_slice_to_lds<S>(x, eb, lb, own, kh);
@@ -478,8 +478,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
                     const double val =
                         (ei == OZ_NONFINITE || fj == OZ_NONFINITE) ? __builtin_nan("") : ldexp(d4[v], ei + fj);
                     double* dst = C + (long)c * p.ldc + row;
-                    const double t0 = beta0 ? 0.0 : (*dst) * p.sc.bd;
-                    *dst = __builtin_fma(val, p.sc.ad, t0);
+                    const double t0 = beta0 ? 0.0 : (*gbl(dst)) * p.sc.bd;
+                    *gbl(dst) = __builtin_fma(val, p.sc.ad, t0);
                 }
             }
         }
