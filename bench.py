@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--secondary", action="store_true")
     ap.add_argument("--grouped", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only time a device-to-device copy of MIB MiB (HBM calibration for the counters)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     return ap.parse_args()
 
@@ -313,6 +315,14 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     stream = torch.cuda.current_stream()
+    if args.copy_probe:
+        n = args.copy_probe << 20
+        src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        ms = time_steps(lambda: dst.copy_(src), args.steps, args.warmup, stream)
+        print(json.dumps({"copy_probe_MiB": args.copy_probe, "ms_per_copy": round(ms, 5),
+                          "read_plus_write_GBs": round(2 * n / (ms * 1e-3) / 1e9, 1)}), flush=True)
+        return
     prob = Problem("i8", 1, 1, 512, 512, 512, 64, alpha=1, beta=args.beta, seed=SEED + 2)
     ms = time_steps(lambda: prob.call(0, stream), args.steps, args.warmup, stream)
     sec = ms * 1e-3
@@ -320,7 +330,7 @@ def main():
     q = prob.bytes()
     achieved_gbs = q / sec / 1e9
     traffic = None
-    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "tile_gemm_i8_TT_512x64_traffic.json")
+    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "dma_stream_gemm_i8_TT_512x64_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
             traffic = json.load(f).get("beta%d" % args.beta)
