@@ -365,13 +365,19 @@ __global__ void __launch_bounds__(256) grouped_gemm(const GroupedDevice d)
     __shared__ __attribute__((aligned(16))) char smem[LA > LB ? LA : LB];
     __shared__ int s_item;
     const int tid = threadIdx.x;
-    for (;;) {
-        if (tid == 0) s_item = (int)atomicAdd(&d.counters[0], 1u);
-        __syncthreads();
-        // wave-uniform, so the item's fields live in scalar registers
-        const int it = __builtin_amdgcn_readfirstlane(s_item);
-        __syncthreads();
-        if (it >= d.nitems) break;
+    if (tid == 0) s_item = (int)atomicAdd(&d.counters[0], 1u);
+    __syncthreads();
+    // wave-uniform, so the item's fields live in scalar registers
+    int it = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    while (it < d.nitems) {
+        // Claim the next item now (its number is read once this one is done,
+        // so the atomic's round trip overlaps this item's work) -- except in
+        // the last two rounds of the queue, where holding a second item would
+        // leave other workgroups idle at the end.
+        const bool early = it + 2 * (int)gridDim.x < d.nitems;
+        unsigned next = 0;
+        if (tid == 0 && early) next = atomicAdd(&d.counters[0], 1u);
         const GItem item = d.items[it];
         const GInst in = d.insts[item.inst];
         const GGroup g = d.groups[in.group];
@@ -393,6 +399,10 @@ __global__ void __launch_bounds__(256) grouped_gemm(const GroupedDevice d)
         j.bkc = g.bkc;
         if (g.form == FA::ID) g_run<FA>(item.cfg, j, in.abytes, in.bbytes, smem);
         else g_run<FB>(item.cfg, j, in.abytes, in.bbytes, smem);
+        if (tid == 0) s_item = (int)(early ? next : atomicAdd(&d.counters[0], 1u));
+        __syncthreads();
+        it = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
     }
     // Every workgroup gets here once; the last one re-arms the counters for
     // the next launch (all other workgroups made their final counter update
