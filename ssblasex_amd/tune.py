@@ -15,6 +15,14 @@ the fastest one for the problem's bucket:
 The table is written to ssblasex_amd/tune/<arch>.txt (or --out) and read by
 the library at its first call (SSBLAS_TUNE_TABLE overrides the path).  The
 library falls back to its size heuristic for buckets the table lacks.
+
+--set grouped tunes the tile shape of grouped plans (g32, g64, g128) per
+(form, size class of m, n, k), size classes <= 32, <= 64, <= 128, larger:
+for every bucket it builds a plan over a few hundred seeded instances drawn
+inside the bucket (random layouts, padded leading dimensions), times its
+execution with each shape forced (SSBLAS_GROUPED_CFG) and records the best
+as a row "form 0 0 cm cn ck 9 gNN us".  Rows of other sets already in the
+table are kept.
 """
 from __future__ import annotations
 
@@ -124,9 +132,98 @@ def time_config(S, torch, form, ta, tb, m, n, k, batch, cfg, steps, warmup):
 time_config.cache = {}
 
 
+GROUPED_CFGS = ["g32", "g64", "g128"]
+GROUPED_WORK_CLASS = 9
+CLASS_RANGE = [(8, 32), (33, 64), (65, 128), (129, 512)]
+
+
+def tune_grouped(S, torch, forms, quick, log):
+    """Best grouped tile shape per (form, size class m, n, k); returns table rows."""
+    import math as _m
+
+    rows = {}
+    rng = np.random.default_rng(0x55B1A5)
+    codes = {"i8": (0, 3, 1, 0), "f32": (4, 4, 1.0, 0.0), "f64": (5, 5, 1.0, 0.0)}
+    for form in forms:
+        for cm in range(4):
+            for cn in range(4):
+                for ck in range(4):
+                    lo_hi = [CLASS_RANGE[c] for c in (cm, cn, ck)]
+                    mean = [_m.sqrt(a * b) for a, b in lo_hi]
+                    count = int(min(2000, max(64, 4e9 / (2 * mean[0] * mean[1] * mean[2]))))
+                    if quick:
+                        count = min(count, 256)
+                    dims = [[int(_m.floor(_m.exp(rng.uniform(_m.log(a), _m.log(b + 1))))) for a, b in lo_hi]
+                            for _ in range(count)]
+                    group = 16
+                    ngroups = (count + group - 1) // group
+                    gsize = [min(group, count - g * group) for g in range(ngroups)]
+                    ta = [int(x) for x in rng.integers(0, 2, ngroups)]
+                    tb = [int(x) for x in rng.integers(0, 2, ngroups)]
+                    ab, cc, al, be = codes[form]
+                    esz = 1 if form == "i8" else (8 if form == "f64" else 4)
+                    osz = 8 if form == "f64" else 4
+                    m_, n_, k_, lda, ldb, ldc, bufs = [], [], [], [], [], [], []
+                    for g in range(ngroups):
+                        for i in range(g * group, g * group + gsize[g]):
+                            m, n, k = dims[i]
+                            ra, ca = (k, m) if ta[g] else (m, k)
+                            rb, cb = (n, k) if tb[g] else (k, n)
+                            pa, pb, pc = (int(x) for x in rng.integers(0, 17, 3))
+                            m_.append(m); n_.append(n); k_.append(k)
+                            lda.append(ra + pa); ldb.append(rb + pb); ldc.append(m + pc)
+                            bufs.append((torch.zeros((ra + pa) * ca * esz, dtype=torch.uint8, device="cuda"),
+                                         torch.zeros((rb + pb) * cb * esz, dtype=torch.uint8, device="cuda"),
+                                         torch.zeros((m + pc) * n * osz, dtype=torch.uint8, device="cuda")))
+                    pA = S.pointer_array([b[0] for b in bufs])
+                    pB = S.pointer_array([b[1] for b in bufs])
+                    pC = S.pointer_array([b[2] for b in bufs])
+                    res = {}
+                    for cfg in GROUPED_CFGS:
+                        os.environ["SSBLAS_GROUPED_CFG"] = cfg
+                        try:
+                            plan = S.GroupedPlan(gsize, ta, tb, [ab] * ngroups, [ab] * ngroups, [cc] * ngroups,
+                                                 [al] * ngroups, [be] * ngroups, m_, n_, k_, pA, lda, pB, ldb, pC,
+                                                 ldc)
+                        finally:
+                            os.environ.pop("SSBLAS_GROUPED_CFG", None)
+                        stream = torch.cuda.current_stream()
+                        for _ in range(3):
+                            plan.execute(stream)
+                        torch.cuda.synchronize()
+                        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+                        steps = 5 if quick else 10
+                        ev[0].record(stream)
+                        for _ in range(steps):
+                            plan.execute(stream)
+                        ev[1].record(stream)
+                        torch.cuda.synchronize()
+                        res[cfg] = ev[0].elapsed_time(ev[1]) * 1000.0 / steps
+                        plan.close()
+                    best = min(res, key=res.get)
+                    rows[(FORMS[form], 0, 0, cm, cn, ck, GROUPED_WORK_CLASS)] = (best, res[best])
+                    log(f"grouped {form} classes m{cm} n{cn} k{ck} ({count} instances): best {best} "
+                        + " ".join(f"{c}={v:.1f}" for c, v in res.items()))
+                    del bufs, pA, pB, pC
+                    torch.cuda.empty_cache()
+    return rows
+
+
+def read_table(path: Path):
+    rows = {}
+    if path.exists():
+        for line in path.read_text().splitlines():
+            if line.startswith("#") or not line.strip():
+                continue
+            f = line.split()
+            if len(f) >= 9:
+                rows[tuple(int(x) for x in f[:7])] = (f[7], float(f[8]))
+    return rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--set", default="all", choices=["bench", "grid", "all"])
+    ap.add_argument("--set", default="all", choices=["bench", "grid", "all", "grouped"])
     ap.add_argument("--out", default=None)
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--steps", type=int, default=20)
@@ -137,10 +234,12 @@ def main(argv=None):
 
     arch = torch.cuda.get_device_properties(0).gcnArchName.split(":")[0]
     out = Path(a.out) if a.out else TABLE_DIR / f"{arch}.txt"
-    rows = {}
+    rows = read_table(out if out.exists() else TABLE_DIR / f"{arch}.txt")
     steps = 5 if a.quick else a.steps
     t0 = time.time()
-    for (form, ta, tb, m, n, k, batch) in problem_sets(a.set):
+    if a.set == "grouped":
+        rows.update(tune_grouped(S, torch, ["i8", "f32", "f64"], a.quick, lambda msg: print(msg, flush=True)))
+    for (form, ta, tb, m, n, k, batch) in (problem_sets(a.set) if a.set != "grouped" else []):
         res = {}
         for cfg in candidates(m, n):
             try:
