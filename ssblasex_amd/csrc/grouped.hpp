@@ -69,7 +69,7 @@ struct GroupedDevice {
 // k-depth of a grouped tile per form and shape.
 constexpr int grouped_bk(int form, int cfg)
 {
-    return form == K_I8 ? 64 : form == K_F64 ? 16 : (form == K_F32 && cfg == GCFG_128) ? 16 : 32;
+    return form == K_I8 ? 64 : form == K_F64 ? 16 : 32;
 }
 // Largest tile shape per form: the bf16 split forms keep two or three
 // operand planes in LDS and fp64 holds twice the accumulator registers, so
