@@ -130,6 +130,7 @@ struct ArithI8 {
     static constexpr int E = 1, NPLANE = 1, LDS_E = 1, KSTEP = 64;
     static constexpr int ID = ARITH_I8;
     static constexpr bool ROW_PERM = false;
+    static constexpr bool SWZ = false;
     __device__ static __forceinline__ void mma(Acc& c, const u32x4* a, const u32x4* b)
     {
         c = __builtin_amdgcn_mfma_i32_16x16x64_i8((i32x4)a[0], (i32x4)b[0], c, 0, 0, 0);
@@ -145,6 +146,7 @@ struct ArithF32 {
     static constexpr int E = 4, NPLANE = 1, LDS_E = 4, KSTEP = 16;
     static constexpr int ID = ARITH_F32;
     static constexpr bool ROW_PERM = false;
+    static constexpr bool SWZ = false;
     __device__ static __forceinline__ void mma(Acc& c, const u32x4* a, const u32x4* b)
     {
         f32x4 fa = (f32x4)a[0], fb = (f32x4)b[0];
@@ -163,6 +165,7 @@ struct ArithF64 {
     static constexpr int E = 8, NPLANE = 1, LDS_E = 8, KSTEP = 8;
     static constexpr int ID = ARITH_F64;
     static constexpr bool ROW_PERM = true;
+    static constexpr bool SWZ = false;
     __device__ static __forceinline__ void mma(Acc& c, const u32x4* a, const u32x4* b)
     {
         f64x2 da = __builtin_bit_cast(f64x2, a[0]);
@@ -183,6 +186,9 @@ struct ArithBF16Split {
     static constexpr int E = 4, NPLANE = NP, LDS_E = 2, KSTEP = 32;
     static constexpr int ID = IDV;
     static constexpr bool ROW_PERM = false;
+    // LDS rows unpadded, 16-byte chunks XOR-swizzled (lds_row_off): three
+    // planes of a 256x128 tile then fit the 160 KiB
+    static constexpr bool SWZ = true;
     __device__ static __forceinline__ void mma(Acc& c, const u32x4* a, const u32x4* b)
     {
         // Smallest terms first so they are not absorbed by the large ones.
@@ -244,6 +250,26 @@ __device__ __forceinline__ int32_t epi(int32_t acc, int32_t c_old, const Scalars
     uint32_t r = (uint32_t)s.ai * (uint32_t)acc;
     if (!beta0) r += (uint32_t)s.bi * (uint32_t)c_old;
     return (int32_t)r;
+}
+
+// Byte offset of (row, kbyte) in an LDS plane of pitch rowb.  Padded rows
+// (SWZ false) carry 16 spare bytes; swizzled rows (SWZ true) are unpadded and
+// 16-byte chunk c of row r sits at chunk c ^ ((r >> 1) & (chunks - 1)), so
+// the 8 rows one ds_read_b128 cycle serves land on distinct banks.
+template <class AR>
+__device__ __forceinline__ int lds_row_off(int row, int kbyte, int rowb)
+{
+    if constexpr (AR::SWZ) {
+        const int mask = (rowb >> 4) - 1;
+        return row * rowb + ((((kbyte >> 4) ^ ((row >> 1) & mask)) << 4) | (kbyte & 15));
+    } else {
+        return row * rowb + kbyte;
+    }
+}
+template <class AR>
+constexpr int lds_row_pitch(int bk)
+{
+    return bk * AR::LDS_E + (AR::SWZ ? 0 : 16);
 }
 
 // Row of the 16-row A fragment that MFMA input lane r (= lane % 16) must

@@ -195,7 +195,7 @@ struct StagerImpl {
             const int kbyte = k0 * AR::LDS_E;
 #pragma unroll
             for (int i = 0; i < RU; ++i) {
-                char* dst = lds + (r0 + i) * rowb + kbyte;
+                char* dst = lds + lds_row_off<AR>(r0 + i, kbyte, rowb);
                 if constexpr (!KC && E == 1) {
                     *reinterpret_cast<u32x2*>(dst) = u32x2{rows[i][0], rows[i][1]};
                 } else if constexpr (AR::LDS_E == E) {
@@ -322,7 +322,7 @@ struct TileJob {
 template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N>
 struct TileShape {
     static constexpr int NT = 64 * WAVES_M * WAVES_N;
-    static constexpr int ROWB = BK * AR::LDS_E + 16;  // LDS row pitch (16 B pad)
+    static constexpr int ROWB = lds_row_pitch<AR>(BK);  // LDS row pitch
     static constexpr int A_PLANE = BM * ROWB;
     static constexpr int B_PLANE = BN * ROWB;
     static constexpr int BUF = AR::NPLANE * (A_PLANE + B_PLANE);
@@ -349,14 +349,14 @@ __device__ __forceinline__ void tile_compute(typename AR::Acc (&acc)[S::FM][S::F
             const int row = wm * S::WTM + fm * 16 + frag_row<AR>(lane & 15);
 #pragma unroll
             for (int pl = 0; pl < AR::NPLANE; ++pl)
-                af[fm][pl] = *reinterpret_cast<const u32x4*>(la + pl * S::A_PLANE + row * S::ROWB + kbyte);
+                af[fm][pl] = *reinterpret_cast<const u32x4*>(la + pl * S::A_PLANE + lds_row_off<AR>(row, kbyte, S::ROWB));
         }
 #pragma unroll
         for (int fn = 0; fn < S::FN; ++fn) {
             const int row = wn * S::WTN + fn * 16 + (lane & 15);
 #pragma unroll
             for (int pl = 0; pl < AR::NPLANE; ++pl)
-                bfg[fn][pl] = *reinterpret_cast<const u32x4*>(lb + pl * S::B_PLANE + row * S::ROWB + kbyte);
+                bfg[fn][pl] = *reinterpret_cast<const u32x4*>(lb + pl * S::B_PLANE + lds_row_off<AR>(row, kbyte, S::ROWB));
         }
 #pragma unroll
         for (int fm = 0; fm < S::FM; ++fm)

@@ -22,7 +22,7 @@ template <class AR, int BM, int BN, int WM, int WN>
 constexpr int fit_k(int bk)
 {
     while (bk > AR::KSTEP &&
-           2 * AR::NPLANE * (BM + BN) * (bk * AR::LDS_E + 16) > 160 * 1024)
+           2 * AR::NPLANE * (BM + BN) * lds_row_pitch<AR>(bk) > 160 * 1024)
         bk /= 2;
     return bk;
 }
@@ -33,7 +33,7 @@ struct CfgShape;
     struct CfgShape<AR, C> {                                                          \
         static constexpr int BM = BM_, BN = BN_, WM = WM_, WN = WN_, STAGES = ST;     \
         static constexpr int BK = fit_k<AR, BM_, BN_, WM_, WN_>(base_k<AR>() * KMUL); \
-        static constexpr bool FITS = 2 * AR::NPLANE * (BM_ + BN_) * (BK * AR::LDS_E + 16) <= 160 * 1024; \
+        static constexpr bool FITS = 2 * AR::NPLANE * (BM_ + BN_) * lds_row_pitch<AR>(BK) <= 160 * 1024; \
     };
 SSB_CFG(CFG_TILE_64, 64, 64, 2, 2, (AR::ID == ARITH_F32 ? 2 : 1), 1)
 SSB_CFG(CFG_TILE_128, 128, 128, 2, 2, 1, 1)
