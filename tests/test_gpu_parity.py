@@ -110,6 +110,9 @@ def test_int8_dma_tiles(cfg, ta, tb):
     check(make_batch("i8", ta, tb, 256, 256, 256, 3, alpha=1, beta=0, seed=61))
     check(make_batch("i8", ta, tb, 256, 256, 384, 2, alpha=-2, beta=1, seed=62))
     check(make_batch("i8", ta, tb, 200, 136, 100, 2, alpha=3, beta=-1, pad=(1, 0, 2), seed=63))
+    # whole and partial tiles in one batch
+    check(make_batch("i8", ta, tb, 320, 256, 128, 2, alpha=1, beta=0, seed=64))
+    check(make_batch("i8", ta, tb, 256, 320, 64, 3, alpha=-1, beta=2, seed=65))
 
 
 @pytest.mark.parametrize("t", ["i8", "f32", "f64"])
