@@ -65,29 +65,32 @@ struct Scalars {
 };
 
 // ---------------------------------------------------------------------------
-// bf16 split of an fp32 value (round to nearest even at every cut).
+// bf16 split of two fp32 values (round to nearest even at every cut).
 // hi = RN_bf16(x), r1 = x - hi (exact in fp32), mid = RN_bf16(r1), ...
-// Non-finite inputs pass through hi unchanged; their tails are set to zero.
+// Each cut is one v_cvt_pk_bf16_f32 (two values, hardware RNE); plane p of
+// the pair comes back as one dword, x0 in the low half.  A non-finite value
+// keeps its hi part (inf stays inf, NaN stays NaN); its tails are NaN, so
+// the C entries it reaches are NaN.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t bf16_rn_bits(float x)
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t bf16_pack_rn(float x0, float x1)
 {
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return u >> 16;  // inf / nan: truncate
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return u >> 16;
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){x0, x1}, bf16x2));
 }
-__device__ __forceinline__ float bf16_bits_to_f32(uint32_t h) { return __uint_as_float(h << 16); }
 
 template <int NP>
-__device__ __forceinline__ void bf16_split(float x, uint32_t (&parts)[NP])
+__device__ __forceinline__ void bf16_split2(float x0, float x1, uint32_t (&parts)[NP])
 {
-    float r = x;
-    bool finite = (__float_as_uint(x) & 0x7f800000u) != 0x7f800000u;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
-        uint32_t h = bf16_rn_bits(r);
-        parts[p] = (p == 0 || finite) ? h : 0u;
-        r = r - bf16_bits_to_f32(h);  // exact (Sterbenz-type cancellation)
+        const uint32_t h = bf16_pack_rn(x0, x1);
+        parts[p] = h;
+        if (p + 1 < NP) {
+            x0 = x0 - __uint_as_float(h << 16);  // exact (Sterbenz-type cancellation)
+            x1 = x1 - __uint_as_float(h & 0xffff0000u);
+        }
     }
 }
 
@@ -208,21 +211,11 @@ struct ArithBF16Split {
     // returned in the low two dwords of planes[p].
     __device__ static __forceinline__ void to_planes(const u32x4& in, u32x4* planes)
     {
-        uint32_t parts[4][NP];
-        const uint32_t w[4] = {in.x, in.y, in.z, in.w};
+        uint32_t lo[NP], hi[NP];
+        bf16_split2<NP>(__uint_as_float(in.x), __uint_as_float(in.y), lo);
+        bf16_split2<NP>(__uint_as_float(in.z), __uint_as_float(in.w), hi);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            float x;
-            __builtin_memcpy(&x, &w[e], 4);
-            bf16_split<NP>(x, parts[e]);
-        }
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-            planes[p][0] = parts[0][p] | (parts[1][p] << 16);
-            planes[p][1] = parts[2][p] | (parts[3][p] << 16);
-            planes[p][2] = 0u;
-            planes[p][3] = 0u;
-        }
+        for (int p = 0; p < NP; ++p) planes[p] = u32x4{lo[p], hi[p], 0u, 0u};
     }
 };
 using ArithBF16X3 = ArithBF16Split<2, 1, ARITH_BF16X3>;
