@@ -115,6 +115,14 @@ __device__ __forceinline__ void wait_vm()
     __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (7 << 4) | (0xF << 8));
 }
 __device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xF | (7 << 4) | (0 << 8) | (3 << 14)); }
+// A value the compiler cannot see through: address arithmetic derived from
+// it is recomputed where it is used instead of being hoisted out of the
+// k-loop and kept live in registers across it.
+__device__ __forceinline__ int opaque(int x)
+{
+    __asm__ volatile("" : "+v"(x));
+    return x;
+}
 // Workgroup barrier that also stops the compiler from moving memory
 // accesses (LDS reads, LDS DMA) across it, without the fence's vmcnt wait.
 __device__ __forceinline__ void lds_barrier()
@@ -142,9 +150,10 @@ struct DmaTile {
     }
 
     // DMA k-block kt of an eligible tile into stage buffer `base`.
-    __device__ __forceinline__ static void issue_dma(const TileJob& j, int kt, char* base, int wave, int lane)
+    __device__ __forceinline__ static void issue_dma(const TileJob& j, int kt, char* base, int wave, int lane_in)
     {
         const int k0 = kt * BK;
+        const int lane = opaque(lane_in);
 #pragma unroll
         for (int q = 0; q < SH::PER_WAVE; ++q) {
             const int inst = wave + q * SH::NWAVE;  // wave instruction index in the stage

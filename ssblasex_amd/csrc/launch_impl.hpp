@@ -69,9 +69,15 @@ struct OccCache {
 // Persistent grid: every tile gets its own workgroup when they all fit at
 // once; otherwise as many workgroups as are resident, a multiple of 8 so
 // that each workgroup keeps one XCD's share of the work ids.
+// SSBLAS_MAX_GRID=<n> caps the grid below that (tests use it to give every
+// workgroup several tiles at small sizes).
 inline long persistent_grid(long work, int per_cu)
 {
     long slots = (long)per_cu * device_cu_count();
+    if (const char* e = getenv("SSBLAS_MAX_GRID")) {
+        const long cap = atol(e);
+        if (cap >= 1 && cap < slots) slots = cap;
+    }
     if (work <= slots) return work;
     return slots >= 8 ? (slots / 8) * 8 : slots;
 }

@@ -115,6 +115,47 @@ def test_int8_dma_tiles(cfg, ta, tb):
     check(make_batch("i8", ta, tb, 256, 320, 64, 3, alpha=-1, beta=2, seed=65))
 
 
+@pytest.mark.parametrize("cfg", ["dma64", "dma128", "dma128s3", "dma256x128"])
+@pytest.mark.parametrize("grid", ["3", "8"])
+@pytest.mark.parametrize("ta,tb", OPS)
+def test_int8_stream_several_tiles_per_workgroup(cfg, grid, ta, tb):
+    """The streaming int8 kernels with the grid capped so that every
+    workgroup walks several whole tiles (the next tile's k-blocks are in
+    flight while a tile's C is written), beta 0 and 1, one instance whose A
+    is not 16-byte aligned in the middle of the batch (the stream stops in
+    front of it, runs it on the guarded path and restarts) and one whose C
+    is not."""
+    import torch
+
+    from ssblasex_amd import pointer_array
+
+    os.environ["SSBLAS_TILE_CONFIG"] = cfg
+    os.environ["SSBLAS_MAX_GRID"] = grid
+    try:
+        check(make_batch("i8", ta, tb, 256, 384, 128, 5, alpha=3, beta=0, seed=66))
+        check(make_batch("i8", ta, tb, 256, 256, 192, 4, alpha=-1, beta=1, seed=67))
+        b = make_batch("i8", ta, tb, 256, 256, 256, 5, alpha=2, beta=0, seed=68)
+        d = DeviceBatch(b)
+        x = d.A[2]
+        big = torch.empty(x.numel() + 1, dtype=x.dtype, device=x.device)
+        big[1:].copy_(x)
+        d.A[2] = big[1:]
+        d.pA = pointer_array(d.A)
+        # and one C 4 bytes off 16-byte alignment (written without vectors)
+        y = d.C[3]
+        bigc = torch.empty(y.numel() + 1, dtype=y.dtype, device=y.device)
+        bigc[1:].copy_(y)
+        d.C[3] = bigc[1:]
+        d.pC = pointer_array(d.C)
+        st, C = run_gpu(b, dev=d)
+        assert st == 0
+        _, R = run_oracle(b, NTHREADS)
+        for i, (x, y) in enumerate(zip(C, R)):
+            assert np.array_equal(x, y), f"instance {i} differs"
+    finally:
+        os.environ.pop("SSBLAS_MAX_GRID", None)
+
+
 @pytest.mark.parametrize("t", ["i8", "f32", "f64"])
 @pytest.mark.parametrize("ta,tb", OPS)
 def test_unaligned_pointers_and_odd_ld(t, ta, tb):
