@@ -16,6 +16,9 @@ Prints one JSON line.  --secondary adds the other benchmark configurations
 (fp32 64^3 x128 NN, fp64 32^3 x65536 NT native vs Ozaki, fp32 256^3 x4096
 TN native vs BF16x3/x6) under "secondary"; --grouped (implied by
 --secondary) adds the grouped 10000-instance uneven batch under "grouped_c5".
+--case ID:FORM times only one of those (c1, c3, c3l, c4 with a form name
+such as native, bf16x6 or ozaki_s9; c5 with all, small or large) and prints
+its own JSON line, for profiler runs.
 """
 from __future__ import annotations
 
@@ -45,6 +48,8 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only time a device-to-device copy of MIB MiB (HBM calibration for the counters)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--case", default=None, metavar="ID:FORM",
+                    help="time only one other configuration, e.g. c4:bf16x6, c3:ozaki_s9, c5:all (profiling runs)")
     return ap.parse_args()
 
 
@@ -176,28 +181,37 @@ def cpu_baseline_i8(prob: Problem, threads: int):
                       f"median of {len(times)} runs, {med:.3f} s each"}
 
 
-def secondary(args, stream):
-    """Other benchmark configurations: native vs split forms, error vs the
-    FP64 oracle measured on a sample of instances."""
+# The other benchmark configurations: (case id, name, problem, forms).
+SECONDARY_ROWS = [
+    ("c1", "fp32 NN m=n=k=64 batch 128 beta=0.5", dict(t="f32", ta=0, tb=0, m=64, n=64, k=64, batch=128, alpha=1.0,
+                                                       beta=0.5, seed=SEED + 1), [("native", 0, "f32")]),
+    ("c3", "fp64 NT m=n=k=32 batch 65536", dict(t="f64", ta=0, tb=1, m=32, n=32, k=32, batch=65536, alpha=1.0,
+                                                beta=0.0, seed=SEED + 3, dist="wide"),
+     [("native", 0, "f64"), ("ozaki_s6", 0x106, "f64"), ("ozaki_s7", 0x107, "f64"), ("ozaki_s8", 0x108, "f64"),
+      ("ozaki_s9", 0x109, "f64")]),
+    ("c3l", "fp64 NN m=n=k=256 batch 512", dict(t="f64", ta=0, tb=0, m=256, n=256, k=256, batch=512, alpha=1.0,
+                                                beta=0.0, seed=SEED + 5, dist="wide"),
+     [("native", 0, "f64"), ("ozaki_s7", 0x107, "f64"), ("ozaki_s8", 0x108, "f64"), ("ozaki_s9", 0x109, "f64")]),
+    ("c4", "fp32 TN m=n=k=256 batch 4096 alpha=1.5 beta=-0.5",
+     dict(t="f32", ta=1, tb=0, m=256, n=256, k=256, batch=4096, alpha=1.5, beta=-0.5, seed=SEED + 4,
+          dist="normal"), [("native", 0, "f32"), ("bf16x3", 4, "bf16"), ("bf16x6", 5, "bf16")]),
+]
+
+
+def secondary(args, stream, only=None):
+    """Other benchmark configurations: native vs split forms.  `only` =
+    (case id, form name) runs just that one (profiling runs)."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import torch
 
     out = {}
-    rows = [
-        ("fp32 NN m=n=k=64 batch 128 beta=0.5", dict(t="f32", ta=0, tb=0, m=64, n=64, k=64, batch=128, alpha=1.0,
-                                                     beta=0.5, seed=SEED + 1), [("native", 0, "f32")]),
-        ("fp64 NT m=n=k=32 batch 65536", dict(t="f64", ta=0, tb=1, m=32, n=32, k=32, batch=65536, alpha=1.0,
-                                              beta=0.0, seed=SEED + 3, dist="wide"),
-         [("native", 0, "f64"), ("ozaki_s6", 0x106, "f64"), ("ozaki_s7", 0x107, "f64"), ("ozaki_s8", 0x108, "f64"),
-          ("ozaki_s9", 0x109, "f64")]),
-        ("fp64 NN m=n=k=256 batch 512", dict(t="f64", ta=0, tb=0, m=256, n=256, k=256, batch=512, alpha=1.0,
-                                             beta=0.0, seed=SEED + 5, dist="wide"),
-         [("native", 0, "f64"), ("ozaki_s7", 0x107, "f64"), ("ozaki_s8", 0x108, "f64"), ("ozaki_s9", 0x109, "f64")]),
-        ("fp32 TN m=n=k=256 batch 4096 alpha=1.5 beta=-0.5",
-         dict(t="f32", ta=1, tb=0, m=256, n=256, k=256, batch=4096, alpha=1.5, beta=-0.5, seed=SEED + 4,
-              dist="normal"), [("native", 0, "f32"), ("bf16x3", 4, "bf16"), ("bf16x6", 5, "bf16")]),
-    ]
-    for name, kw, forms in rows:
+    for cid, name, kw, forms in SECONDARY_ROWS:
+        if only is not None:
+            if cid != only[0]:
+                continue
+            forms = [f for f in forms if f[0] == only[1]]
+            if not forms:
+                raise SystemExit(f"case {cid} has no form {only[1]}")
         p = Problem(**kw)
         res = {}
         for fname, ct, peak_key in forms:
@@ -216,7 +230,7 @@ def secondary(args, stream):
     return out
 
 
-def grouped_c5(args, stream):
+def grouped_c5(args, stream, only=None):
     """Grouped uneven batch: 10000 instances, m, n, k = floor(exp(U[ln 8, ln 513))),
     ld = dim + U{0..16}, transa/transb drawn per group of 256, groups alternately
     fp32 N(0,1) and int8 U{-128..127}, alpha = 1, beta = 0, buffers allocated in a
@@ -295,11 +309,14 @@ def grouped_c5(args, stream):
                 "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "fp32_mfma_frac_if_fp32_alone": round(flops["f32"] / (ms * 1e-3) / 1e12 / PEAK["f32"], 4)}
 
-    out = {"all": run(range(count))}
     big = [i for i in range(count) if min(inst[i][3:6]) >= 256]
     small = [i for i in range(count) if max(inst[i][3:6]) <= 32]
-    out["small (max dim <= 32)"] = run(small)
-    out["large (min dim >= 256)"] = run(big)
+    subsets = {"all": ("all", range(count)), "small": ("small (max dim <= 32)", small),
+               "large": ("large (min dim >= 256)", big)}
+    out = {}
+    for key, (label, sel) in subsets.items():
+        if only is None or only == key:
+            out[label] = run(sel)
     del bufs
     torch.cuda.empty_cache()
     return out
@@ -322,6 +339,13 @@ def main():
         ms = time_steps(lambda: dst.copy_(src), args.steps, args.warmup, stream)
         print(json.dumps({"copy_probe_MiB": args.copy_probe, "ms_per_copy": round(ms, 5),
                           "read_plus_write_GBs": round(2 * n / (ms * 1e-3) / 1e9, 1)}), flush=True)
+        return
+    if args.case:
+        cid, _, form = args.case.partition(":")
+        if cid == "c5":
+            print(json.dumps({"case": args.case, "result": grouped_c5(args, stream, only=form or "all")}), flush=True)
+        else:
+            print(json.dumps({"case": args.case, "result": secondary(args, stream, only=(cid, form))}), flush=True)
         return
     prob = Problem("i8", 1, 1, 512, 512, 512, 64, alpha=1, beta=args.beta, seed=SEED + 2)
     ms = time_steps(lambda: prob.call(0, stream), args.steps, args.warmup, stream)
