@@ -31,6 +31,8 @@ const char* config_name(int cfg)
     case CFG_DMA_128_K2: return "dma128k2";
     case CFG_DMA_256x128: return "dma256x128";
     case CFG_DMA_128_S3: return "dma128s3";
+    case CFG_KC_256x128: return "kc256x128";
+    case CFG_KC_128: return "kc128";
     default: return "unknown";
     }
 }
@@ -40,7 +42,8 @@ void config_tile(int cfg, int* tm, int* tn)
     static const int dims[CFG_COUNT][2] = {{16, 16},   {32, 32},   {64, 64},   {128, 128}, {128, 64},
                                            {64, 128},  {128, 128}, {128, 128}, {256, 128}, {64, 64},
                                            {128, 128}, {128, 128}, {128, 128}, {256, 128}, {64, 64},
-                                           {128, 128}, {128, 128}, {256, 128}, {128, 128}};
+                                           {128, 128}, {128, 128}, {256, 128}, {128, 128},
+                                           {256, 128}, {128, 128}};
     if (cfg < 0 || cfg >= CFG_COUNT) cfg = CFG_TILE_64;
     *tm = dims[cfg][0];
     *tn = dims[cfg][1];

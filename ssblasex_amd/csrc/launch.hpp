@@ -39,7 +39,12 @@ enum TileConfig : int {
     CFG_DMA_128_K2 = 16,    // 128x128, k 128, 3 stages
     CFG_DMA_256x128 = 17,   // 256x128, 8 waves, k 64, 3 stages
     CFG_DMA_128_S3 = 18,    // 128x128, k 64, 3 stages
-    CFG_COUNT = 19
+    // fp32 / fp64 / bf16 forms with K-contiguous operands (A^T, B): direct-to-
+    // LDS loads of the input elements, bf16 split in registers (int8 uses
+    // DMA_128; other layouts the TILE config of the same shape)
+    CFG_KC_256x128 = 19,    // 256x128, 8 waves, 128 bytes of k per row, 3 stages
+    CFG_KC_128 = 20,        // 128x128, 4 waves, 128 bytes of k per row, 2 stages
+    CFG_COUNT = 21
 };
 
 const char* config_name(int cfg);

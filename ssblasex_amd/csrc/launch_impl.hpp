@@ -4,6 +4,7 @@
 #pragma once
 #include <stdlib.h>
 #include "gemm_dma.hpp"
+#include "gemm_dma_kc.hpp"
 #include "gemm_kernels.hpp"
 #include "launch.hpp"
 
@@ -127,6 +128,34 @@ hipError_t run_dma(KParams p, hipStream_t st)
     }
 }
 
+// Direct-to-LDS tiles of K-contiguous fp32 / fp64 operands.  HCFG: the
+// register-staged config of the same shape, used for other layouts and for
+// problems whose tiles are not whole.  WPE: waves per SIMD to fit.
+template <class AR, bool AKC, bool BKC, int BM, int BN, int WM, int WN, int S, int HCFG, int WPE>
+hipError_t run_kc(KParams p, hipStream_t st)
+{
+    if constexpr (AR::ID == ARITH_I8) {
+        return run_dma<AR, AKC, BKC, BM, BN, 64, WM, WN, 3>(p, st);  // int8 has its own direct-to-LDS tiles
+    } else if constexpr (!(AKC && BKC)) {
+        return run_tile<AR, AKC, BKC, HCFG>(p, st);
+    } else {
+        constexpr int E = AR::E;
+        constexpr int BK = 128 / E;  // 128 bytes of k per row and stage
+        const bool whole = p.m % BM == 0 && p.n % BN == 0 && p.k % BK == 0 && (p.lda * E) % 16 == 0 &&
+                           (p.ldb * E) % 16 == 0;
+        if (!whole) return run_tile<AR, AKC, BKC, HCFG>(p, st);
+        constexpr int NT = 64 * WM * WN;
+        auto kern = dma_kc_gemm<AR, BM, BN, BK, WM, WN, S, base_k<AR>(), WPE>;
+        p.tiles_m = p.m / BM;
+        p.tiles_n = p.n / BN;
+        p.total = (long)p.batch * p.tiles_m * p.tiles_n;
+        int occ = OccCache<decltype(kern)>::get(kern, NT);
+        long grid = persistent_grid(p.total, occ);
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), 0, st, p);
+        return hipGetLastError();
+    }
+}
+
 template <class AR, bool AKC, bool BKC, int W>
 hipError_t run_direct(KParams p, hipStream_t st)
 {
@@ -164,6 +193,8 @@ hipError_t run_cfg(const KParams& p, int cfg, hipStream_t st)
     case CFG_DMA_128_K2: return run_dma<AR, AKC, BKC, 128, 128, 128, 2, 2, 3>(p, st);
     case CFG_DMA_256x128: return run_dma<AR, AKC, BKC, 256, 128, 64, 4, 2, 3>(p, st);
     case CFG_DMA_128_S3: return run_dma<AR, AKC, BKC, 128, 128, 64, 2, 2, 3>(p, st);
+    case CFG_KC_256x128: return run_kc<AR, AKC, BKC, 256, 128, 4, 2, 3, CFG_TILE_256x128, 2>(p, st);
+    case CFG_KC_128: return run_kc<AR, AKC, BKC, 128, 128, 2, 2, 2, CFG_TILE_128, 2>(p, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -308,34 +308,33 @@ __global__ void __launch_bounds__(256) ozaki_gemm(const OzParams p)
 
 // ---------------------------------------------------------------------------
 // fused form for small instances (m, n, k <= 32): one wave per instance,
-// nothing in HBM but A, B and C.
+// nothing in HBM but A, B and C, and no LDS.
 //
 // Slicing: lane l owns row l % 32 of op(A) and column l % 32 of op(B) over
 // the k half kh = l / 32 (16 doubles each); the row / column maximum joins
-// the two halves with one lane exchange.  Each slice of 16 values is one
-// 16-byte chunk, written to the wave's LDS area
-//     [operand][slice p][row 0..31][2 chunks of 16 k]
-// with the chunk index XOR-ed with bit 2 of the row, so the 16-lane
-// fragment reads below touch distinct banks.
+// the two halves with one lane exchange.  Each slice of the 16 values is
+// one 16-byte register chunk.
 //
-// Products: one v_mfma_i32_16x16x64_i8 consumes k = 64: lane groups 0, 1
-// (lanes 0..31) carry the 32 k of slice pair (p, t-p) and groups 2, 3 those
-// of pair (p+1, t-p-1), so each MFMA adds two products of the same order t,
-// whose sum is what D_t needs.  D_t is exact in int32 (2 * 9 * 32 * 127^2 <
-// 2^31) and the fold, exponents and epilogue are ozaki_gemm's, so the
-// result is bit-identical to the general path.
+// Products: v_mfma_i32_32x32x32_i8 takes its A operand as lane l -> row
+// l % 32, 16 consecutive k of half l / 32, and B likewise with columns --
+// exactly the chunk the lane sliced -- so one instruction is the whole
+// 32x32x32 product of one slice pair, straight from registers.  Its result
+// is lane l -> column l % 32, accumulator v -> row 8*(v/4) + 4*(l/32) + v%4.
+// D_t = sum of the pairs of order t is exact in int32 (9 * 32 * 127^2 <
+// 2^31); the fold (smallest order first), the exponents and the epilogue
+// are ozaki_gemm's, so the result is bit-identical to the general path.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int oz_chunk_off(int p, int row, int half)
-{
-    return p * 1024 + row * 32 + 16 * (half ^ ((row >> 2) & 1));
-}
+typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
 
+// Consumes v (scaled and cut in place) into S slice chunks.  Each slice is
+// packed before the next is cut (the empty asm makes the remainders depend
+// on it): left alone, the compiler cuts all slices first and keeps S x 16
+// doubles live.
 template <int S>
-__device__ __forceinline__ void oz_slice_to_lds(const double* x, int e, char* lds, int row, int half)
+__device__ __forceinline__ void oz_slice_regs(double* v, int e, u32x4 (&sl)[S])
 {
-    double v[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = (e == OZ_NONFINITE) ? 0.0 : ldexp(x[i], -e);
+    for (int i = 0; i < 16; ++i) v[i] = (e == OZ_NONFINITE) ? 0.0 : ldexp(v[i], -e);
 #pragma unroll
     for (int p = 0; p < S; ++p) {
         uint32_t w[4] = {0u, 0u, 0u, 0u};
@@ -346,7 +345,10 @@ __device__ __forceinline__ void oz_slice_to_lds(const double* x, int e, char* ld
             v[i] -= t;
             w[i >> 2] |= ((uint32_t)(int32_t)t & 0xffu) << (8 * (i & 3));
         }
-        *reinterpret_cast<u32x4*>(lds + oz_chunk_off(p, row, half)) = u32x4{w[0], w[1], w[2], w[3]};
+        __asm__ volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]), "+v"(w[3]));
+#pragma unroll
+        for (int i = 0; i < 16; ++i) __asm__ volatile("" : "+v"(v[i]));
+        sl[p] = u32x4{w[0], w[1], w[2], w[3]};
     }
 }
 
@@ -366,130 +368,181 @@ __device__ __forceinline__ int oz_exp16(const double* x)
     return exp_of_max(mx, bad);
 }
 
-template <int S>
+// Where a lane's 16 doubles sit in one operand: row `own` of op(A) (column
+// `own` of op(B)) over the k half kh.  Indices are clamped into the matrix,
+// so every load is in bounds and unconditional; elements outside the
+// instance are zeroed afterwards (only when it is not a full 32 x 32 block).
+struct OzLane {
+    uint32_t base, step;  // element offset of k = 16 kh and per k (both < 2^32: a 32x32 block of one instance)
+    uint32_t kvalid;      // elements of the half inside the instance (0..16)
+    bool row_ok;
+};
+__device__ __forceinline__ OzLane oz_lane(long ld, bool kcontig, int rows, int k, int own, int kh)
+{
+    OzLane L;
+    const uint32_t r = (uint32_t)min(own, rows - 1);
+    const uint32_t k0 = (uint32_t)min(16 * kh, k - 1);
+    L.step = kcontig ? 1u : (uint32_t)ld;
+    L.base = kcontig ? r * (uint32_t)ld + k0 : r + k0 * (uint32_t)ld;
+    L.kvalid = (uint32_t)max(0, min(16, k - 16 * kh));
+    L.row_ok = own < rows;
+    return L;
+}
+// A pointer read from a pointer array at a wave-uniform index, as a scalar
+// (the compiler cannot use a scalar load for it: the kernel's stores might
+// alias the array), so element addresses are scalar base + 32-bit offset.
+template <class T>
+__device__ __forceinline__ T* wave_uniform(T* ptr)
+{
+    const uint64_t v = reinterpret_cast<uint64_t>(ptr);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+// FULL: the instance is a whole 32 x 32 x 32 block (no element to zero).
+template <bool FULL>
+__device__ __forceinline__ void oz_small_load(const void* X, const OzLane& L, double (&x)[16])
+{
+    const double* P = wave_uniform(static_cast<const double*>(X));
+    if constexpr (FULL) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) x[q] = *gbl(P + (L.base + (uint32_t)q * L.step));
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const uint32_t qq = (uint32_t)q < L.kvalid ? (uint32_t)q : 0u;  // stays inside the matrix
+            const double v = *gbl(P + (L.base + qq * L.step));
+            x[q] = (L.row_ok && (uint32_t)q < L.kvalid) ? v : 0.0;
+        }
+    }
+}
+
+// Persistent: wave w of the grid computes instances w, w + W, w + 2W, ...
+// (W = waves in the grid).  As soon as an operand of the current instance
+// is sliced, its registers are refilled with the next instance's, so those
+// loads are in flight during the slice products, the fold and the epilogue.
+template <int S, bool FULL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) ozaki_small(const OzParams p)
 {
-    __shared__ __attribute__((aligned(16))) char smem[4][2 * S * 1024];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int inst = blockIdx.x * 4 + wave;
+    const int lane = threadIdx.x & 63;
+    // wave index as a scalar: instance numbers, and so the operand pointers
+    // read from the pointer arrays, are then known to be wave-uniform
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long W = (long)gridDim.x * 4;
+    long inst = (long)blockIdx.x * 4 + wave;
     if (inst >= p.batch) return;
-    char* la = smem[wave];
-    char* lb = la + S * 1024;
-    const double* A = static_cast<const double*>(p.A[inst]);
-    const double* B = static_cast<const double*>(p.B[inst]);
-
-    // slices of row (lane % 32) of op(A) and column (lane % 32) of op(B), k half lane / 32
     const int own = lane & 31, kh = lane >> 5;
-    int ea, eb;
-    {
-        double x[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int kk = kh * 16 + q;
-            x[q] = 0.0;
-            if (own < p.m && kk < p.k) x[q] = *gbl(A + (p.ta ? kk + (long)own * p.lda : own + (long)kk * p.lda));
-        }
-        ea = oz_exp16(x);
-        oz_slice_to_lds<S>(x, ea, la, own, kh);
-    }
-    {
-        double x[16];
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int kk = kh * 16 + q;
-            x[q] = 0.0;
-            if (own < p.n && kk < p.k) x[q] = *gbl(B + (p.tb ? own + (long)kk * p.ldb : kk + (long)own * p.ldb));
-        }
-        eb = oz_exp16(x);
-        oz_slice_to_lds<S>(x, eb, lb, own, kh);
-    }
-    // the wave's own LDS writes are read back by other lanes of the wave
-    __asm__ volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-
-    const int r = lane & 15, g = lane >> 4, half = g & 1;
-    const bool hi = g >= 2;
-    f64x4 D[2][2];
-#pragma unroll
-    for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-        for (int fn = 0; fn < 2; ++fn) D[fm][fn] = f64x4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int t = S + 1; t >= 2; --t) {
-        const int plo = t - S > 1 ? t - S : 1, phi = S < t - 1 ? S : t - 1;
-        i32x4 acc[2][2];
-#pragma unroll
-        for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < 2; ++fn) acc[fm][fn] = i32x4{0, 0, 0, 0};
-#pragma unroll
-        for (int pa = plo; pa <= phi; pa += 2) {
-            const int pb = pa + 1;  // second pair of the step, when pb <= phi
-            const bool two = pb <= phi;
-            // slice numbers (1-based) this lane feeds: pair a in groups 0, 1, pair b in 2, 3
-            const int sa_ = hi ? pb : pa, sb_ = t - sa_;
-            u32x4 af[2], bf[2];
-#pragma unroll
-            for (int f = 0; f < 2; ++f) {
-                const int row = f * 16 + r;
-                if (!hi || two) {
-                    af[f] = *reinterpret_cast<const u32x4*>(la + oz_chunk_off(sa_ - 1, row, half));
-                    bf[f] = *reinterpret_cast<const u32x4*>(lb + oz_chunk_off(sb_ - 1, row, half));
-                } else {
-                    af[f] = u32x4{0u, 0u, 0u, 0u};
-                    bf[f] = u32x4{0u, 0u, 0u, 0u};
-                }
-            }
-#pragma unroll
-            for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-                for (int fn = 0; fn < 2; ++fn) ArithI8::mma(acc[fm][fn], &af[fm], &bf[fn]);
-        }
-        const double scale = ldexp(1.0, -7 * t);
-#pragma unroll
-        for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < 2; ++fn) {
-                int32_t a4[4];
-                __builtin_memcpy(a4, &acc[fm][fn], 16);
-                double d4[4];
-                __builtin_memcpy(d4, &D[fm][fn], 32);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) d4[v] = __builtin_fma((double)a4[v], scale, d4[v]);
-                __builtin_memcpy(&D[fm][fn], d4, 32);
-            }
-    }
-
-    // epilogue: C = fma(D * 2^(e_i + f_j), alpha, fl(beta * C))
-    double* C = static_cast<double*>(p.C[inst]);
     const bool beta0 = p.beta0 != 0;
+    // op(A) row own is K-contiguous when A is transposed; op(B) column own
+    // when B is not
+    const OzLane LA = oz_lane(p.lda, p.ta != 0, p.m, p.k, own, kh);
+    const OzLane LB = oz_lane(p.ldb, p.tb == 0, p.n, p.k, own, kh);
+
+    for (; inst < p.batch; inst += W) {
+        double xa[16], xb[16];
+        oz_small_load<FULL>(p.A[inst], LA, xa);
+        oz_small_load<FULL>(p.B[inst], LB, xb);
+        u32x4 sa[S], sb[S];
+        const int ea = oz_exp16(xa);
+        oz_slice_regs<S>(xa, ea, sa);
+        const int eb = oz_exp16(xb);
+        oz_slice_regs<S>(xb, eb, sb);
+
+        double D[16];
 #pragma unroll
-    for (int fm = 0; fm < 2; ++fm)
+        for (int v = 0; v < 16; ++v) D[v] = 0.0;
 #pragma unroll
-        for (int fn = 0; fn < 2; ++fn) {
-            const int c = fn * 16 + r;
-            const int fj = __shfl(eb, c);  // lane c holds column c
-            double d4[4];
-            __builtin_memcpy(d4, &D[fm][fn], 32);
+        for (int t = S + 1; t >= 2; --t) {
+            const int plo = t - S > 1 ? t - S : 1, phi = S < t - 1 ? S : t - 1;
+            i32x16 acc = {};
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int row = fm * 16 + 4 * g + v;
-                const int ei = __shfl(ea, row);  // lane `row` holds that row
-                if (row < p.m && c < p.n) {
-                    const double val =
-                        (ei == OZ_NONFINITE || fj == OZ_NONFINITE) ? __builtin_nan("") : ldexp(d4[v], ei + fj);
-                    double* dst = C + (long)c * p.ldc + row;
-                    const double t0 = beta0 ? 0.0 : (*gbl(dst)) * p.sc.bd;
-                    *gbl(dst) = __builtin_fma(val, p.sc.ad, t0);
+            for (int pa = plo; pa <= phi; ++pa)
+                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8((i32x4)sa[pa - 1], (i32x4)sb[t - pa - 1], acc, 0, 0, 0);
+            const double scale = ldexp(1.0, -7 * t);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) D[v] = __builtin_fma((double)acc[v], scale, D[v]);
+            // fold each order here: left alone, the compiler sinks the folds
+            // into the epilogue and keeps S accumulators of 16 registers live
+#pragma unroll
+            for (int v = 0; v < 16; ++v) __asm__ volatile("" : "+v"(D[v]));
+        }
+
+        // epilogue: C = fma(D * 2^(e_i + f_j), alpha, fl(beta * C)); the
+        // lane's 4 consecutive rows of its column go out as two 16-byte
+        // stores when they are whole and aligned
+        double* C = wave_uniform(static_cast<double*>(p.C[inst]));
+        const bool vec = (p.ldc % 2) == 0 && aligned16(C);
+        const int c = own;
+        // row exponents, shuffled while every lane is active (lane `row`
+        // holds that row's)
+        int eir[16];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) eir[v] = __shfl(ea, 8 * (v >> 2) + 4 * kh + (v & 3));
+        if (c < p.n) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row0 = 8 * q + 4 * kh;
+                double val[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int ei = eir[4 * q + v];
+                    val[v] = (ei == OZ_NONFINITE || eb == OZ_NONFINITE) ? __builtin_nan("")
+                                                                        : ldexp(D[4 * q + v], ei + eb);
+                }
+                double* dst = C + (long)c * p.ldc + row0;
+                if (vec && row0 + 4 <= p.m) {
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        f64x2 old = {0.0, 0.0};
+                        if (!beta0) old = *gbl(reinterpret_cast<const f64x2*>(dst + 2 * h));
+                        f64x2 res;
+                        res[0] = __builtin_fma(val[2 * h], p.sc.ad, beta0 ? 0.0 : old[0] * p.sc.bd);
+                        res[1] = __builtin_fma(val[2 * h + 1], p.sc.ad, beta0 ? 0.0 : old[1] * p.sc.bd);
+                        *gbl(reinterpret_cast<f64x2*>(dst + 2 * h)) = res;
+                    }
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        if (row0 + v < p.m) {
+                            const double t0 = beta0 ? 0.0 : (*gbl(dst + v)) * p.sc.bd;
+                            *gbl(dst + v) = __builtin_fma(val[v], p.sc.ad, t0);
+                        }
+                    }
                 }
             }
         }
+    }
+}
+
+template <int S, bool FULL>
+static hipError_t run_ozaki_small_k(const OzParams& p, hipStream_t st)
+{
+    static int occ[16] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
+    if (occ[dev] == 0) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ozaki_small<S, FULL>, 256, 0) != hipSuccess || nb < 1)
+            nb = 1;
+        occ[dev] = nb;
+    }
+    const long blocks = ((long)p.batch + 3) / 4;
+    long grid = (long)occ[dev] * device_cu_count();
+    if (const char* e = getenv("SSBLAS_MAX_GRID")) {
+        const long cap = atol(e);
+        if (cap >= 1 && cap < grid) grid = cap;
+    }
+    if (blocks < grid) grid = blocks;
+    hipLaunchKernelGGL((ozaki_small<S, FULL>), dim3((unsigned)grid), dim3(256), 0, st, p);
+    return hipGetLastError();
 }
 
 template <int S>
 static hipError_t run_ozaki_small(const OzParams& p, hipStream_t st)
 {
-    hipLaunchKernelGGL(ozaki_small<S>, dim3((unsigned)((p.batch + 3) / 4)), dim3(256), 0, st, p);
-    return hipGetLastError();
+    if (p.m == 32 && p.n == 32 && p.k == 32) return run_ozaki_small_k<S, true>(p, st);
+    return run_ozaki_small_k<S, false>(p, st);
 }
 
 static hipError_t ozaki_small_dispatch(const OzParams& p, hipStream_t st)

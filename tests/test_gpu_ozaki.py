@@ -100,10 +100,18 @@ def test_ozaki_config_fp64_32cubed_sample():
 
 @pytest.mark.parametrize("ta,tb", OPS)
 @pytest.mark.parametrize("s", [4, 9])
-def test_ozaki_fused_small_equals_general(ta, tb, s, monkeypatch):
+@pytest.mark.parametrize("shape,pad,grid", [((32, 29, 31), (0, 3, 1), None), ((32, 29, 31), (0, 3, 1), "3"),
+                                            ((32, 32, 32), (0, 0, 2), "3"), ((28, 32, 32), (2, 0, 4), None)])
+def test_ozaki_fused_small_equals_general(ta, tb, s, shape, pad, grid, monkeypatch):
     """Instances with m, n, k <= 32 run the fused one-wave kernel; it must
-    give the general (workspace) path's result bit for bit."""
-    b = make_batch("f64", ta, tb, 32, 29, 31, 40, alpha=-1.5, beta=0.25, pad=(0, 3, 1), seed=11 + s, dist="wide")
+    give the general (workspace) path's result bit for bit, also when every
+    wave walks several instances (grid capped: the next instance's operands
+    are loaded while the current one is computed) and when C is written as
+    16-byte pairs (even ldc) or element by element (odd ldc)."""
+    m, n, k = shape
+    if grid is not None:
+        monkeypatch.setenv("SSBLAS_MAX_GRID", grid)
+    b = make_batch("f64", ta, tb, m, n, k, 40, alpha=-1.5, beta=0.25, pad=pad, seed=11 + s, dist="wide")
     st, fused = run_gpu(b, compute_type=0x100 + s)
     assert st == 0
     monkeypatch.setenv("SSBLAS_OZAKI_NO_FUSED", "1")

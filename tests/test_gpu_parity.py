@@ -29,7 +29,7 @@ SHAPES = [(1, 1, 1), (7, 5, 3), (16, 16, 64), (33, 17, 65), (64, 64, 64), (65, 6
           (130, 257, 200)]
 CONFIGS = ["direct16", "direct32", "tile64", "tile128", "tile128x64", "tile64x128", "tile128k2", "tile128w8",
            "tile256x128", "tile64s2", "tile128s2", "tile128k2s2", "tile128w8s2", "tile256x128s2", "dma64",
-           "dma128", "dma128k2", "dma256x128", "dma128s3"]
+           "dma128", "dma128k2", "dma256x128", "dma128s3", "kc256x128", "kc128"]
 
 
 @pytest.fixture(autouse=True)
@@ -152,6 +152,49 @@ def test_int8_stream_several_tiles_per_workgroup(cfg, grid, ta, tb):
         _, R = run_oracle(b, NTHREADS)
         for i, (x, y) in enumerate(zip(C, R)):
             assert np.array_equal(x, y), f"instance {i} differs"
+    finally:
+        os.environ.pop("SSBLAS_MAX_GRID", None)
+
+
+KC_FORMS = [("f32", 0, 2.0), ("f64", 0, 2.0), ("bf16x6", 5, 4.0), ("bf16x3", 4, None)]
+
+
+@pytest.mark.parametrize("cfg", ["kc256x128", "kc128"])
+@pytest.mark.parametrize("form,ct,factor", KC_FORMS, ids=[f[0] for f in KC_FORMS])
+@pytest.mark.parametrize("grid", ["3", "0"])
+def test_kc_dma_tiles(cfg, form, ct, factor, grid):
+    """Direct-to-LDS tiles of K-contiguous fp operands (op(A) = A^T, op(B) =
+    B) in every form: whole tiles streamed through capped and uncapped grids,
+    beta 0 and nonzero, and one instance whose A is one element off alignment (its
+    tiles run as register-staged sub-tiles inside the same launch)."""
+    import torch
+
+    from ssblasex_amd import pointer_array
+
+    t = "f64" if form == "f64" else "f32"
+    bar = (lambda e, r: e <= 2.0 ** -14) if factor is None else (lambda e, r: e <= fp_bar(t, r, factor))
+    os.environ["SSBLAS_TILE_CONFIG"] = cfg
+    if grid != "0":
+        os.environ["SSBLAS_MAX_GRID"] = grid
+    try:
+        for (m, n, k, batch, alpha, beta, seed) in [(256, 384, 256, 3, 1.5, -0.5, 71), (256, 256, 128, 2, 1.0, 0.0, 72)]:
+            b = make_batch(t, 1, 0, m, n, k, batch, alpha=alpha, beta=beta, seed=seed, dist="normal")
+            st, C = run_gpu(b, compute_type=ct)
+            assert st == 0
+            _, R = run_oracle(b, NTHREADS)
+            assert bar(normwise(b, C), normwise(b, R)), (m, n, k, normwise(b, C), normwise(b, R))
+        b = make_batch(t, 1, 0, 256, 256, 256, 4, alpha=0.75, beta=0.5, seed=73, dist="normal")
+        d = DeviceBatch(b)
+        x = d.A[1]
+        big = torch.empty(x.numel() + 1, dtype=x.dtype, device=x.device)
+        big[1:].copy_(x)
+        d.A[1] = big[1:]
+        d.pA = pointer_array(d.A)
+        st, C = run_gpu(b, compute_type=ct, dev=d)
+        assert st == 0
+        _, R = run_oracle(b, NTHREADS)
+        # normwise() is the worst instance, the misaligned one included
+        assert bar(normwise(b, C), normwise(b, R)), (normwise(b, C), normwise(b, R))
     finally:
         os.environ.pop("SSBLAS_MAX_GRID", None)
 
