@@ -614,7 +614,10 @@ direct_gemm(const KParams p)
 {
     const int per_inst = p.tiles_m * p.tiles_n;
     const long nwaves = (long)gridDim.x * 4;
-    for (long t = (long)blockIdx.x * 4 + (threadIdx.x >> 6); t < p.total; t += nwaves) {
+    // the wave index as a scalar, so the work id, and the instance pointers
+    // read with it, are known to be the same in every lane
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (long t = (long)blockIdx.x * 4 + wave; t < p.total; t += nwaves) {
         const int inst = (int)(t / per_inst);
         const int rem = (int)(t % per_inst);
         const TileJob j = make_job(p, inst, (rem % p.tiles_m) * WTM, (rem / p.tiles_m) * WTN);

@@ -326,29 +326,67 @@ __global__ void __launch_bounds__(256) ozaki_gemm(const OzParams p)
 // ---------------------------------------------------------------------------
 typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
 
-// Consumes v (scaled and cut in place) into S slice chunks.  Each slice is
-// packed before the next is cut (the empty asm makes the remainders depend
-// on it): left alone, the compiler cuts all slices first and keeps S x 16
-// doubles live.
-template <int S>
-__device__ __forceinline__ void oz_slice_regs(double* v, int e, u32x4 (&sl)[S])
+// The 7-bit digits of a 28-bit value, one per byte: byte 3 the most
+// significant digit, byte 0 the least.
+__device__ __forceinline__ uint32_t oz_spread7(uint32_t h)
 {
+    return (h & 0x7fu) | ((h << 1) & 0x7f00u) | ((h << 2) & 0x7f0000u) | ((h << 3) & 0x7f000000u);
+}
+
+// Slices of 16 values (v = x * 2^-e, |v| < 1) into S 16-byte chunks: slice
+// p holds, for every value, its p-th base-128 digit with the value's sign
+// (the digits the sequence "v *= 128; d = trunc(v); v -= d" gives, as in
+// ozaki_slice).  Done in integers: H = floor(|v| 2^28) carries digits 1-4,
+// L = floor(frac(|v| 2^28) 2^28) digits 5-8 and floor(frac(|v| 2^56) 2^7)
+// digit 9 (every step exact in fp64); the 7-bit fields are spread to bytes
+// and transposed across 4 values; a negative value's bytes are negated at
+// once (bytes below 128: -d = ((0x80 - d) ^ 0x80) without borrows).
+template <int S>
+__device__ __forceinline__ void oz_slice_regs(const double* x, int e, u32x4 (&sl)[S])
+{
+    static_assert(S >= 1 && S <= 9, "1 to 9 slices");
+    uint32_t H[16], L[16], N[16], neg[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-    for (int i = 0; i < 16; ++i) v[i] = (e == OZ_NONFINITE) ? 0.0 : ldexp(v[i], -e);
-#pragma unroll
-    for (int p = 0; p < S; ++p) {
-        uint32_t w[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            v[i] *= 128.0;
-            const double t = trunc(v[i]);
-            v[i] -= t;
-            w[i >> 2] |= ((uint32_t)(int32_t)t & 0xffu) << (8 * (i & 3));
+    for (int i = 0; i < 16; ++i) {
+        const double a = (e == OZ_NONFINITE) ? 0.0 : ldexp(fabs(x[i]), -e);
+        neg[i >> 2] |= (x[i] < 0.0 ? 0xffu : 0u) << (8 * (i & 3));
+        const double y1 = ldexp(a, 28);
+        H[i] = (uint32_t)y1;
+        if constexpr (S > 4) {
+            const double y2 = ldexp(y1 - (double)H[i], 28);
+            L[i] = (uint32_t)y2;
+            if constexpr (S > 8) N[i] = (uint32_t)ldexp(y2 - (double)L[i], 7);
         }
-        __asm__ volatile("" : "+v"(w[0]), "+v"(w[1]), "+v"(w[2]), "+v"(w[3]));
+    }
 #pragma unroll
-        for (int i = 0; i < 16; ++i) __asm__ volatile("" : "+v"(v[i]));
-        sl[p] = u32x4{w[0], w[1], w[2], w[3]};
+    for (int j = 0; j < 4; ++j) {
+        uint32_t w[9];
+        {
+            const uint32_t in[4] = {oz_spread7(H[4 * j]), oz_spread7(H[4 * j + 1]), oz_spread7(H[4 * j + 2]),
+                                    oz_spread7(H[4 * j + 3])};
+            uint32_t out[4];
+            transpose4x4_bytes(in, out);  // out[c]: byte c of the four values
+            w[0] = out[3];
+            w[1] = out[2];
+            w[2] = out[1];
+            w[3] = out[0];
+        }
+        if constexpr (S > 4) {
+            const uint32_t in[4] = {oz_spread7(L[4 * j]), oz_spread7(L[4 * j + 1]), oz_spread7(L[4 * j + 2]),
+                                    oz_spread7(L[4 * j + 3])};
+            uint32_t out[4];
+            transpose4x4_bytes(in, out);
+            w[4] = out[3];
+            w[5] = out[2];
+            w[6] = out[1];
+            w[7] = out[0];
+        }
+        if constexpr (S > 8) w[8] = N[4 * j] | (N[4 * j + 1] << 8) | (N[4 * j + 2] << 16) | (N[4 * j + 3] << 24);
+#pragma unroll
+        for (int p = 0; p < S; ++p) {
+            const uint32_t nw = (0x80808080u - w[p]) ^ 0x80808080u;
+            sl[p][j] = (neg[j] & nw) | (~neg[j] & w[p]);
+        }
     }
 }
 
@@ -419,9 +457,10 @@ __device__ __forceinline__ void oz_small_load(const void* X, const OzLane& L, do
 }
 
 // Persistent: wave w of the grid computes instances w, w + W, w + 2W, ...
-// (W = waves in the grid).  As soon as an operand of the current instance
-// is sliced, its registers are refilled with the next instance's, so those
-// loads are in flight during the slice products, the fold and the epilogue.
+// (W = waves in the grid), loading each instance's operands straight into
+// registers.  (Copying the next instance global -> LDS while computing the
+// current one was measured slower: the extra LDS reads and waits cost more
+// than the load latency they hide at two waves per SIMD.)
 template <int S, bool FULL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) ozaki_small(const OzParams p)
 {

@@ -119,3 +119,34 @@ def test_ozaki_fused_small_equals_general(ta, tb, s, shape, pad, grid, monkeypat
     assert st == 0
     for x, y in zip(fused, general):
         assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("ta,tb", OPS)
+@pytest.mark.parametrize("pad", [(0, 0, 0), (1, 0, 0), (0, 2, 1)], ids=str)
+def test_ozaki_fused_full_blocks(ta, tb, pad, monkeypatch):
+    """Whole 32 x 32 x 32 instances (the fused kernel's unguarded loads):
+    even and odd leading dimensions, one instance 8 bytes off 16-byte
+    alignment, every wave walking several instances (grid capped).
+    Bit-identical to the general path."""
+    import torch
+
+    from helpers import DeviceBatch
+    from ssblasex_amd import pointer_array
+
+    monkeypatch.setenv("SSBLAS_MAX_GRID", "2")
+    b = make_batch("f64", ta, tb, 32, 32, 32, 24, alpha=0.5, beta=-1.0, pad=pad, seed=97, dist="wide")
+    d = DeviceBatch(b)
+    x = d.A[5]
+    big = torch.empty(x.numel() + 1, dtype=x.dtype, device=x.device)
+    big[1:].copy_(x)
+    d.A[5] = big[1:]
+    d.pA = pointer_array(d.A)
+    st, fused = run_gpu(b, compute_type=0x109, dev=d)
+    assert st == 0
+    monkeypatch.setenv("SSBLAS_OZAKI_NO_FUSED", "1")
+    d.reset_c()
+    st, general = run_gpu(b, compute_type=0x109, dev=d)
+    assert st == 0
+    for i, (x, y) in enumerate(zip(fused, general)):
+        assert np.array_equal(x, y), f"instance {i}"
+
