@@ -9,9 +9,11 @@
 
 namespace ssb {
 
-// Tile shapes of the grouped kernel (4 waves in a 2x2 grid, LDS
-// double-buffered, operand layouts read per item at run time).
-enum GroupedCfg : int { GCFG_32 = 0, GCFG_64 = 1, GCFG_128 = 2, GCFG_COUNT = 3 };
+// Tile shapes of the grouped kernel.  32, 64, 128: workgroup tiles (4 waves
+// in a 2x2 grid, LDS double-buffered, operand layouts read per item at run
+// time).  W32: one wave per 32x32 block, operands loaded straight into MFMA
+// registers (no LDS, no barriers); each wave of a workgroup takes its own.
+enum GroupedCfg : int { GCFG_32 = 0, GCFG_64 = 1, GCFG_128 = 2, GCFG_W32 = 3, GCFG_COUNT = 4 };
 
 // Kernel families: the arithmetic forms one grouped kernel can run.  int8
 // shares a kernel with whichever fp32 form the plan uses, so the usual
@@ -53,16 +55,18 @@ struct GItem {
 };
 
 struct GroupedDevice {
-    const GItem* items;  // sorted by estimated cost, longest first
-    int nitems;
+    const GItem* items;  // workgroup items, then wave items; each part sorted by estimated cost, longest first
+    int nitems;          // workgroup items
+    int nwitems;         // wave items (items[nitems .. nitems + nwitems))
     const GInst* insts;
     const GGroup* groups;
     const void* const* A;  // the caller's device arrays of device pointers
     const void* const* B;
     void* const* C;
-    // two counters, zero between launches: [0] next item, [1] workgroups done.
-    // The last workgroup of a launch resets both, so launches of one plan
-    // must be ordered (one stream, or a graph replayed on one stream).
+    // three counters, zero between launches: [0] next workgroup item, [1]
+    // workgroups done, [2] next wave item.  The last workgroup of a launch
+    // resets them, so launches of one plan must be ordered (one stream, or a
+    // graph replayed on one stream).
     unsigned* counters;
 };
 
@@ -79,7 +83,7 @@ constexpr int grouped_max_cfg(int form)
     return (form == K_BF16X3 || form == K_BF16X6 || form == K_F64) ? GCFG_64 : GCFG_128;
 }
 
-// Names of the grouped tile shapes in the tuning table ("g32", "g64", "g128").
+// Names of the grouped tile shapes in the tuning table ("g32", "g64", "g128", "w32").
 const char* grouped_cfg_name(int cfg);
 // Work-class column value that marks grouped rows in the tuning table.
 constexpr int GROUPED_WORK_CLASS = 9;

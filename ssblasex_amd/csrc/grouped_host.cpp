@@ -27,7 +27,7 @@ namespace ssb {
 
 const char* grouped_cfg_name(int cfg)
 {
-    static const char* names[GCFG_COUNT] = {"g32", "g64", "g128"};
+    static const char* names[GCFG_COUNT] = {"g32", "g64", "g128", "w32"};
     return cfg >= 0 && cfg < GCFG_COUNT ? names[cfg] : "?";
 }
 
@@ -43,7 +43,7 @@ hipError_t launch_grouped(int family, const GroupedDevice& d, hipStream_t st)
 
 namespace {
 
-int tile_of(int cfg) { return cfg == GCFG_32 ? 32 : cfg == GCFG_64 ? 64 : 128; }
+int tile_of(int cfg) { return cfg == GCFG_32 || cfg == GCFG_W32 ? 32 : cfg == GCFG_64 ? 64 : 128; }
 
 int elem_bytes_in(int form) { return form == K_I8 ? 1 : form == K_F64 ? 8 : 4; }
 int elem_bytes_out(int form) { return form == K_F64 ? 8 : 4; }
@@ -54,6 +54,12 @@ int elem_bytes_out(int form) { return form == K_F64 ? 8 : 4; }
 // for the first load and the epilogue.
 double item_ns(int form, int cfg, long k, bool beta0)
 {
+    if (cfg == GCFG_W32) {
+        // one wave of the four a workgroup runs at once: a quarter of the
+        // workgroup tile's rate, no staging latency per k-block
+        const double per_cu = 4.0 / 2.0;  // waves per CU running at once / workgroup tile ratio
+        return (item_ns(form, GCFG_32, k, beta0) - 1000.0) * per_cu + 500.0;
+    }
     // per-CU dense MFMA rate in ops/ns at 2.4 GHz, split forms at their product count
     static const double rate[5] = {8192 * 2.4, 256 * 2.4, 128 * 2.4, 4096 * 2.4 / 3, 4096 * 2.4 / 6};
     const double cu_bw = 8000.0 / 256.0;  // bytes/ns per CU with every CU busy
@@ -71,16 +77,38 @@ long tiles(int cfg, long m, long n)
     return ((m + t - 1) / t) * ((n + t - 1) / t);
 }
 
+// SSBLAS_GROUPED_CFG forces the shape (tuning and test runs): "<shape>" for
+// every instance, or "<shape>:<shape>" for instances with m, n <= 32 and
+// for the others.
+int forced_cfg(long m, long n)
+{
+    const char* e = getenv("SSBLAS_GROUPED_CFG");
+    if (e == nullptr) return -1;
+    char name[16];
+    const char* colon = strchr(e, ':');
+    if (colon != nullptr) {
+        const bool small = m <= 32 && n <= 32;
+        const char* src = small ? e : colon + 1;
+        const size_t len = small ? (size_t)(colon - e) : strlen(colon + 1);
+        if (len >= sizeof(name)) return -1;
+        memcpy(name, src, len);
+        name[len] = 0;
+    } else {
+        if (strlen(e) >= sizeof(name)) return -1;
+        strcpy(name, e);
+    }
+    for (int c = 0; c < GCFG_COUNT; ++c)
+        if (strcmp(name, grouped_cfg_name(c)) == 0) return c;
+    return -1;
+}
+
 int pick_cfg(int form, long m, long n, long k, bool beta0)
 {
-    const char* forced = getenv("SSBLAS_GROUPED_CFG");
     int top = grouped_max_cfg(form);
-    if (forced) {
-        for (int c = 0; c < GCFG_COUNT; ++c)
-            if (strcmp(forced, grouped_cfg_name(c)) == 0) return std::min(c, top);
-    }
+    const int forced = forced_cfg(m, n);
+    if (forced >= 0) return forced == GCFG_W32 ? forced : std::min(forced, top);
     int tuned = grouped_tuned_config(form, m, n, k);
-    if (tuned >= 0) return std::min(tuned, top);
+    if (tuned >= 0) return tuned == GCFG_W32 ? tuned : std::min(tuned, top);
     int best = GCFG_32;
     double best_ns = 1e300;
     for (int c = GCFG_32; c <= top; ++c) {
@@ -90,6 +118,7 @@ int pick_cfg(int form, long m, long n, long k, bool beta0)
             best = c;
         }
     }
+    if (tiles(GCFG_W32, m, n) * item_ns(form, GCFG_W32, k, beta0) < best_ns) best = GCFG_W32;
     return best;
 }
 
@@ -207,9 +236,9 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
     if (!device_accessible(A) || !device_accessible(B) || !device_accessible(C))
         return SSBLAS_STATUS_INVALID_VALUE;
 
-    // Work list per family, longest items first.
+    // Work lists per family (workgroup items, wave items), longest first.
     const int other_family = fp32_form == K_BF16X3 ? GF_I8_BF16X3 : fp32_form == K_BF16X6 ? GF_I8_BF16X6 : GF_I8_F32;
-    std::vector<HostItem> lists[GF_COUNT];
+    std::vector<HostItem> lists[GF_COUNT], wlists[GF_COUNT];
     for (long i = 0; i < total; ++i) {
         const GInst& I = insts[(size_t)i];
         if (I.m == 0 || I.n == 0 || I.k == 0) continue;
@@ -217,15 +246,18 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
         const int cfg = pick_cfg(G.form, I.m, I.n, I.k, G.beta0 != 0);
         const int t = tile_of(cfg);
         const double ns = item_ns(G.form, cfg, I.k, G.beta0 != 0);
-        auto& L = lists[G.form == K_F64 ? GF_F64 : other_family];
+        const int f = G.form == K_F64 ? GF_F64 : other_family;
+        auto& L = cfg == GCFG_W32 ? wlists[f] : lists[f];
         for (int j0 = 0; j0 < I.n; j0 += t)
             for (int i0 = 0; i0 < I.m; i0 += t) L.push_back(HostItem{GItem{(int)i, i0, j0, cfg}, ns});
     }
 
     long nitems = 0;
-    for (auto& L : lists) {
-        std::stable_sort(L.begin(), L.end(), [](const HostItem& a, const HostItem& b) { return a.ns > b.ns; });
-        nitems += (long)L.size();
+    auto by_cost = [](const HostItem& a, const HostItem& b) { return a.ns > b.ns; };
+    for (int f = 0; f < GF_COUNT; ++f) {
+        std::stable_sort(lists[f].begin(), lists[f].end(), by_cost);
+        std::stable_sort(wlists[f].begin(), wlists[f].end(), by_cost);
+        nitems += (long)lists[f].size() + (long)wlists[f].size();
     }
     if (nitems > INT_MAX) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
 
@@ -251,21 +283,24 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
     size_t pos = off_items;
     for (int f = 0; f < GF_COUNT; ++f) {
         auto& L = lists[f];
-        if (L.empty()) continue;
+        auto& WL = wlists[f];
+        if (L.empty() && WL.empty()) continue;
         GItem* dst = reinterpret_cast<GItem*>(image.data() + pos);
         for (size_t i = 0; i < L.size(); ++i) dst[i] = L[i].it;
+        for (size_t i = 0; i < WL.size(); ++i) dst[L.size() + i] = WL[i].it;
         GroupedDevice& D = p->fam[p->nfam];
         D.items = reinterpret_cast<const GItem*>(dev + pos);
         D.nitems = (int)L.size();
+        D.nwitems = (int)WL.size();
         D.insts = reinterpret_cast<const GInst*>(dev + off_insts);
         D.groups = reinterpret_cast<const GGroup*>(dev + off_groups);
         D.A = A;
         D.B = B;
         D.C = C;
-        D.counters = reinterpret_cast<unsigned*>(dev) + 2 * f;
+        D.counters = reinterpret_cast<unsigned*>(dev) + 4 * f;
         p->fam_id[p->nfam] = f;
         ++p->nfam;
-        pos += L.size() * sizeof(GItem);
+        pos += (L.size() + WL.size()) * sizeof(GItem);
     }
     e = hipMemcpy(p->mem, image.data(), bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {

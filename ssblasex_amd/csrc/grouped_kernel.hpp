@@ -357,9 +357,46 @@ __device__ __forceinline__ void g_run(int cfg, const TileJob& j, uint32_t abytes
     gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem);
 }
 
+// One wave's 32x32 block of a grouped instance (W32 items), operand layouts
+// chosen at run time.
+template <class AR>
+__device__ __forceinline__ void g_run_wave(const TileJob& j)
+{
+    if (j.akc) {
+        if (j.bkc) direct_job<AR, true, true, 32, 32>(j);
+        else direct_job<AR, true, false, 32, 32>(j);
+    } else {
+        if (j.bkc) direct_job<AR, false, true, 32, 32>(j);
+        else direct_job<AR, false, false, 32, 32>(j);
+    }
+}
+
+__device__ __forceinline__ TileJob g_job(const GroupedDevice& d, const GItem& item, const GInst& in, const GGroup& g)
+{
+    TileJob j;
+    j.A = static_cast<const char*>(d.A[item.inst]);
+    j.B = static_cast<const char*>(d.B[item.inst]);
+    j.C = d.C[item.inst];
+    j.lda = in.lda;
+    j.ldb = in.ldb;
+    j.ldc = in.ldc;
+    j.m = in.m;
+    j.n = in.n;
+    j.k = in.k;
+    j.i0 = item.i0;
+    j.j0 = item.j0;
+    j.s = Scalars{g.s.af, g.s.bf, g.s.ad, g.s.bd, g.s.ai, g.s.bi};
+    j.beta0 = g.beta0;
+    j.akc = g.akc;
+    j.bkc = g.bkc;
+    return j;
+}
+
 // FA: the family's first form; FB: its second (== FA for one-form families).
+// Workgroup items first (the longer ones), then wave items: each wave of a
+// workgroup takes W32 items from their own counter until none is left.
 template <class FA, class FB>
-__global__ void __launch_bounds__(256) grouped_gemm(const GroupedDevice d)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) grouped_gemm(const GroupedDevice d)
 {
     constexpr int LA = g_lds_max<FA>(), LB = g_lds_max<FB>();
     __shared__ __attribute__((aligned(16))) char smem[LA > LB ? LA : LB];
@@ -381,22 +418,7 @@ __global__ void __launch_bounds__(256) grouped_gemm(const GroupedDevice d)
         const GItem item = d.items[it];
         const GInst in = d.insts[item.inst];
         const GGroup g = d.groups[in.group];
-        TileJob j;
-        j.A = static_cast<const char*>(d.A[item.inst]);
-        j.B = static_cast<const char*>(d.B[item.inst]);
-        j.C = d.C[item.inst];
-        j.lda = in.lda;
-        j.ldb = in.ldb;
-        j.ldc = in.ldc;
-        j.m = in.m;
-        j.n = in.n;
-        j.k = in.k;
-        j.i0 = item.i0;
-        j.j0 = item.j0;
-        j.s = Scalars{g.s.af, g.s.bf, g.s.ad, g.s.bd, g.s.ai, g.s.bi};
-        j.beta0 = g.beta0;
-        j.akc = g.akc;
-        j.bkc = g.bkc;
+        const TileJob j = g_job(d, item, in, g);
         if (g.form == FA::ID) g_run<FA>(item.cfg, j, in.abytes, in.bbytes, smem);
         else g_run<FB>(item.cfg, j, in.abytes, in.bbytes, smem);
         if (tid == 0) s_item = (int)(early ? next : atomicAdd(&d.counters[0], 1u));
@@ -404,8 +426,25 @@ __global__ void __launch_bounds__(256) grouped_gemm(const GroupedDevice d)
         it = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
     }
+    if (d.nwitems > 0) {
+        const int lane = tid & 63;
+        auto claim = [&]() {
+            unsigned v = 0u;
+            if (lane == 0) v = atomicAdd(&d.counters[2], 1u);
+            return __builtin_amdgcn_readfirstlane((int)v);  // lane 0's claim, for the whole wave
+        };
+        for (int w = claim(); w < d.nwitems; w = claim()) {
+            const GItem item = d.items[d.nitems + w];
+            const GInst in = d.insts[item.inst];
+            const GGroup g = d.groups[in.group];
+            const TileJob j = g_job(d, item, in, g);
+            if (g.form == FA::ID) g_run_wave<FA>(j);
+            else g_run_wave<FB>(j);
+        }
+    }
+    __syncthreads();  // every wave of the workgroup is done with its items
     // Every workgroup gets here once; the last one re-arms the counters for
-    // the next launch (all other workgroups made their final counter update
+    // the next launch (all other workgroups made their final counter updates
     // before their fenced arrival).
     if (tid == 0) {
         __threadfence();
@@ -413,6 +452,7 @@ __global__ void __launch_bounds__(256) grouped_gemm(const GroupedDevice d)
         if (arrived == gridDim.x - 1) {
             __threadfence();
             atomicExch(&d.counters[0], 0u);
+            atomicExch(&d.counters[2], 0u);
             atomicExch(&d.counters[1], 0u);
         }
     }
@@ -421,7 +461,7 @@ __global__ void __launch_bounds__(256) grouped_gemm(const GroupedDevice d)
 template <class FA, class FB>
 hipError_t run_grouped(const GroupedDevice& d, hipStream_t st)
 {
-    if (d.nitems <= 0) return hipSuccess;
+    if (d.nitems + d.nwitems <= 0) return hipSuccess;
     auto kern = grouped_gemm<FA, FB>;
     static int occ[16] = {0};
     int dev = 0;
@@ -432,7 +472,8 @@ hipError_t run_grouped(const GroupedDevice& d, hipStream_t st)
         occ[dev] = nb;
     }
     const long slots = (long)occ[dev] * device_cu_count();
-    const long grid = d.nitems < slots ? d.nitems : slots;
+    const long want = (long)d.nitems + ((long)d.nwitems + 3) / 4;  // four waves take wave items at once
+    const long grid = want < slots ? want : slots;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, d);
     return hipGetLastError();
 }

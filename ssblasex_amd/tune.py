@@ -16,7 +16,7 @@ The table is written to ssblasex_amd/tune/<arch>.txt (or --out) and read by
 the library at its first call (SSBLAS_TUNE_TABLE overrides the path).  The
 library falls back to its size heuristic for buckets the table lacks.
 
---set grouped tunes the tile shape of grouped plans (g32, g64, g128) per
+--set grouped tunes the tile shape of grouped plans (g32, g64, g128, w32) per
 (form, size class of m, n, k), size classes <= 32, <= 64, <= 128, larger:
 for every bucket it builds a plan over a few hundred seeded instances drawn
 inside the bucket (random layouts, padded leading dimensions), times its
@@ -132,7 +132,7 @@ def time_config(S, torch, form, ta, tb, m, n, k, batch, cfg, steps, warmup):
 time_config.cache = {}
 
 
-GROUPED_CFGS = ["g32", "g64", "g128"]
+GROUPED_CFGS = ["g32", "g64", "g128", "w32"]
 GROUPED_WORK_CLASS = 9
 CLASS_RANGE = [(8, 32), (33, 64), (65, 128), (129, 512)]
 

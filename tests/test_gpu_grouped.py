@@ -81,8 +81,11 @@ def test_grouped_beta_zero_ignores_nan():
     check(G, got)
 
 
-@pytest.mark.parametrize("cfg", ["g32", "g64", "g128"])
+@pytest.mark.parametrize("cfg", ["g32", "g64", "g128", "w32", "w32:g64", "w32:g128", "g32:w32"])
 def test_grouped_every_tile_shape(cfg, monkeypatch):
+    """Each tile shape forced (w32: one wave per 32x32 block, taken by the
+    waves one at a time after the workgroup tiles), and queues that mix
+    wave items and workgroup items ("small:rest")."""
     monkeypatch.setenv("SSBLAS_GROUPED_CFG", cfg)
     G = make_grouped(48, seed=0x55B1A5 + 53, hi=300, group=8)
     dg = DeviceGrouped(G)
@@ -106,7 +109,10 @@ def test_grouped_edges():
     check(G, got)
 
 
-def test_grouped_fp64_and_bf16x3_families():
+@pytest.mark.parametrize("cfg", [None, "w32", "w32:g64"])
+def test_grouped_fp64_and_bf16x3_families(cfg, monkeypatch):
+    if cfg is not None:
+        monkeypatch.setenv("SSBLAS_GROUPED_CFG", cfg)
     G = make_grouped(60, seed=0x55B1A5 + 55, hi=140, group=10, types=("f64", "f32", "i8"))
     dg = DeviceGrouped(G)
     with plan_for(dg, compute_type=BF16X3) as p:
@@ -114,11 +120,15 @@ def test_grouped_fp64_and_bf16x3_families():
         check(G, dg.results(), fp32_form="bf16x3")
 
 
-def test_grouped_graph_replays():
+@pytest.mark.parametrize("cfg", [None, "w32:g64"])
+def test_grouped_graph_replays(cfg, monkeypatch):
     """Captured graph replayed with eager work between replays: each replay
-    matches the oracle on the data it saw."""
+    matches the oracle on the data it saw (the work counters, wave items'
+    included, are back at zero after every launch)."""
     import torch
 
+    if cfg is not None:
+        monkeypatch.setenv("SSBLAS_GROUPED_CFG", cfg)
     G = make_grouped(64, seed=0x55B1A5 + 56, hi=200)
     dg = DeviceGrouped(G)
     with plan_for(dg) as p:
