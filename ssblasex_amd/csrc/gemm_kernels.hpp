@@ -304,6 +304,63 @@ __device__ __forceinline__ void store_frag(typename AR::Out* C, long ldc, int r0
     }
 }
 
+// The old C values of one fragment (the lane's 4 rows of column c), read
+// before the k-loop of a small tile when beta != 0, so that their latency
+// overlaps the operand loads instead of following the last MFMA.  Zero
+// outside the matrix.
+template <class AR>
+__device__ __forceinline__ void load_old_frag(const typename AR::Out* C, long ldc, int r0, int c, bool fast, int m,
+                                              int n, typename AR::Out (&old)[4])
+{
+    using Out = typename AR::Out;
+    const Out* src = C + (long)c * ldc + r0;
+    if (fast) {
+        if constexpr (sizeof(Out) == 4) {
+            const u32x4 t = *gbl(reinterpret_cast<const u32x4*>(src));
+            const uint32_t w[4] = {t.x, t.y, t.z, t.w};
+            __builtin_memcpy(old, w, 16);
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const f64x2 t = *gbl(reinterpret_cast<const f64x2*>(src + 2 * h));
+                old[2 * h] = t[0];
+                old[2 * h + 1] = t[1];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) old[v] = (c < n && r0 + v < m) ? *gbl(src + v) : Out(0);
+    }
+}
+
+// Epilogue of one fragment whose old C values were read ahead (beta != 0).
+template <class AR>
+__device__ __forceinline__ void store_frag_old(typename AR::Out* C, long ldc, int r0, int c,
+                                               const typename AR::Acc& acc, const Scalars& sc, bool fast, int m, int n,
+                                               const typename AR::Out (&old)[4])
+{
+    using Out = typename AR::Out;
+    Out* dst = C + (long)c * ldc + r0;
+    Out res[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) res[v] = epi(acc[v], old[v], sc, false);
+    if (fast) {
+        if constexpr (sizeof(Out) == 4) {
+            uint32_t w[4];
+            __builtin_memcpy(w, res, 16);
+            *gbl(reinterpret_cast<u32x4*>(dst)) = u32x4{w[0], w[1], w[2], w[3]};
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) *gbl(reinterpret_cast<f64x2*>(dst + 2 * h)) = f64x2{res[2 * h], res[2 * h + 1]};
+        }
+    } else {
+        if (c >= n) return;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            if (r0 + v < m) *gbl(dst + v) = res[v];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // One output tile of one instance: what a work item hands to a tile routine.
 // ---------------------------------------------------------------------------
@@ -415,6 +472,20 @@ __device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
 #pragma unroll
         for (int fn = 0; fn < S::FN; ++fn) acc[fm][fn] = typename AR::Acc{};
 
+    // small wave tiles read the old C values (beta != 0) before the k-loop
+    constexpr bool PRE = S::FM * S::FN <= 4;
+    Out* C = static_cast<Out*>(j.C);
+    const bool c_fast = rows_a == BM && rows_b == BN && ((j.ldc * (long)sizeof(Out)) % 16) == 0 && aligned16(C);
+    Out old[PRE ? S::FM : 1][PRE ? S::FN : 1][4];
+    if (PRE && j.beta0 == 0) {
+#pragma unroll
+        for (int fm = 0; fm < (PRE ? S::FM : 1); ++fm)
+#pragma unroll
+            for (int fn = 0; fn < (PRE ? S::FN : 1); ++fn)
+                load_old_frag<AR>(C, j.ldc, j.i0 + wm * S::WTM + fm * 16 + 4 * (lane >> 4),
+                                  j.j0 + wn * S::WTN + fn * 16 + (lane & 15), c_fast, j.m, j.n, old[fm][fn]);
+    }
+
     auto load_step = [&](int kt, SA& xa, SB& xb) {
         const int k0 = kt * BK;
         const int kv = j.k - k0;
@@ -470,14 +541,18 @@ __device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
     }
 
     // fused alpha/beta epilogue
-    Out* C = static_cast<Out*>(j.C);
-    const bool c_fast = rows_a == BM && rows_b == BN && ((j.ldc * (long)sizeof(Out)) % 16) == 0 && aligned16(C);
 #pragma unroll
     for (int fm = 0; fm < S::FM; ++fm)
 #pragma unroll
         for (int fn = 0; fn < S::FN; ++fn) {
             const int r0 = j.i0 + wm * S::WTM + fm * 16 + 4 * (lane >> 4);
             const int c = j.j0 + wn * S::WTN + fn * 16 + (lane & 15);
+            if constexpr (PRE) {
+                if (j.beta0 == 0) {
+                    store_frag_old<AR>(C, j.ldc, r0, c, acc[fm][fn], j.s, c_fast, j.m, j.n, old[fm][fn]);
+                    continue;
+                }
+            }
             store_frag<AR>(C, j.ldc, r0, c, acc[fm][fn], j.s, j.beta0 != 0, c_fast, j.m, j.n);
         }
 }
@@ -582,6 +657,20 @@ __device__ __forceinline__ void direct_job(const TileJob& j)
 #pragma unroll
         for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = typename AR::Acc{};
 
+    // old C values (beta != 0) read before the k-loop
+    Out* C = static_cast<Out*>(j.C);
+    const bool c_fast = j.i0 + WTM <= j.m && j.j0 + WTN <= j.n && ((j.ldc * (long)sizeof(Out)) % 16) == 0 &&
+                        aligned16(C);
+    Out old[FM][FN][4];
+    if (j.beta0 == 0) {
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn)
+                load_old_frag<AR>(C, j.ldc, j.i0 + fm * 16 + 4 * (lane >> 4), j.j0 + fn * 16 + (lane & 15), c_fast,
+                                  j.m, j.n, old[fm][fn]);
+    }
+
     for (int kb = 0; kb < j.k; kb += AR::KSTEP) {
         u32x4 af[FM][AR::NPLANE], bfg[FN][AR::NPLANE];
 #pragma unroll
@@ -597,14 +686,15 @@ __device__ __forceinline__ void direct_job(const TileJob& j)
             for (int fn = 0; fn < FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
     }
 
-    Out* C = static_cast<Out*>(j.C);
-    const bool c_fast = j.i0 + WTM <= j.m && j.j0 + WTN <= j.n && ((j.ldc * (long)sizeof(Out)) % 16) == 0 &&
-                        aligned16(C);
 #pragma unroll
     for (int fm = 0; fm < FM; ++fm)
 #pragma unroll
         for (int fn = 0; fn < FN; ++fn)
-            store_frag<AR>(C, j.ldc, j.i0 + fm * 16 + 4 * (lane >> 4), j.j0 + fn * 16 + (lane & 15), acc[fm][fn],
+            if (j.beta0 == 0)
+                store_frag_old<AR>(C, j.ldc, j.i0 + fm * 16 + 4 * (lane >> 4), j.j0 + fn * 16 + (lane & 15),
+                                   acc[fm][fn], j.s, c_fast, j.m, j.n, old[fm][fn]);
+            else
+                store_frag<AR>(C, j.ldc, j.i0 + fm * 16 + 4 * (lane >> 4), j.j0 + fn * 16 + (lane & 15), acc[fm][fn],
                            j.s, j.beta0 != 0, c_fast, j.m, j.n);
 }
 
