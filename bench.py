@@ -13,12 +13,13 @@ HIP events on the launch stream and device synchronisation.
                     [--no-cpu-baseline] [--secondary] [--grouped]
 
 Prints one JSON line.  --secondary adds the other benchmark configurations
-(fp32 64^3 x128 NN, fp64 32^3 x65536 NT native vs Ozaki, fp32 256^3 x4096
-TN native vs BF16x3/x6) under "secondary"; --grouped (implied by
+(fp32 64^3 x128 NN, fp64 32^3 x65536 NT native vs Ozaki, fp64 256^3 x512,
+fp32 256^3 x4096 TN native vs BF16x3/x6 with ld 256 and 264, the headline
+batch with beta = 1) under "secondary"; --grouped (implied by
 --secondary) adds the grouped 10000-instance uneven batch under "grouped_c5".
---case ID:FORM times only one of those (c1, c3, c3l, c4 with a form name
-such as native, bf16x6 or ozaki_s9; c5 with all, small or large) and prints
-its own JSON line, for profiler runs.
+--case ID:FORM times only one of those (c1, c3, c3l, c4, c4p, c2b1 with a
+form name such as native, bf16x6 or ozaki_s9; c5 with all, small or large)
+and prints its own JSON line, for profiler runs.
 """
 from __future__ import annotations
 
@@ -195,6 +196,11 @@ SECONDARY_ROWS = [
     ("c4", "fp32 TN m=n=k=256 batch 4096 alpha=1.5 beta=-0.5",
      dict(t="f32", ta=1, tb=0, m=256, n=256, k=256, batch=4096, alpha=1.5, beta=-0.5, seed=SEED + 4,
           dist="normal"), [("native", 0, "f32"), ("bf16x3", 4, "bf16"), ("bf16x6", 5, "bf16")]),
+    ("c4p", "fp32 TN m=n=k=256 batch 4096 alpha=1.5 beta=-0.5, lda=ldb=ldc=264",
+     dict(t="f32", ta=1, tb=0, m=256, n=256, k=256, batch=4096, alpha=1.5, beta=-0.5, seed=SEED + 4,
+          dist="normal", pad=8), [("native", 0, "f32"), ("bf16x3", 4, "bf16"), ("bf16x6", 5, "bf16")]),
+    ("c2b1", "int8 TT m=n=k=512 batch 64 beta=1",
+     dict(t="i8", ta=1, tb=1, m=512, n=512, k=512, batch=64, alpha=1, beta=1, seed=SEED + 2), [("native", 0, "i8")]),
 ]
 
 
@@ -221,7 +227,7 @@ def secondary(args, stream, only=None):
                 res[fname] = {"error": str(e)}
                 continue
             v = p.work() / (ms * 1e-3) / 1e12
-            res[fname] = {"TFLOPs": round(v, 3), "ms_per_step": round(ms, 4),
+            res[fname] = {"TFLOPs" if kw["t"] != "i8" else "TOPS": round(v, 3), "ms_per_step": round(ms, 4),
                           "frac_of_native_peak": round(v / PEAK[kw["t"] if kw["t"] != "i8" else "int8"], 4),
                           "hbm_GBs": round(p.bytes() / (ms * 1e-3) / 1e9, 1)}
         out[name] = res
