@@ -33,6 +33,8 @@ const char* config_name(int cfg)
     case CFG_DMA_128_S3: return "dma128s3";
     case CFG_KC_256x128: return "kc256x128";
     case CFG_KC_128: return "kc128";
+    case CFG_DMA_256: return "dma256";
+    case CFG_DMA_256_W16: return "dma256w16";
     default: return "unknown";
     }
 }
@@ -43,7 +45,7 @@ void config_tile(int cfg, int* tm, int* tn)
                                            {64, 128},  {128, 128}, {128, 128}, {256, 128}, {64, 64},
                                            {128, 128}, {128, 128}, {128, 128}, {256, 128}, {64, 64},
                                            {128, 128}, {128, 128}, {256, 128}, {128, 128},
-                                           {256, 128}, {128, 128}};
+                                           {256, 128}, {128, 128}, {256, 256}, {256, 256}};
     if (cfg < 0 || cfg >= CFG_COUNT) cfg = CFG_TILE_64;
     *tm = dims[cfg][0];
     *tn = dims[cfg][1];

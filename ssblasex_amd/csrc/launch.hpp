@@ -44,7 +44,11 @@ enum TileConfig : int {
     // DMA_128; other layouts the TILE config of the same shape)
     CFG_KC_256x128 = 19,    // 256x128, 8 waves, 128 bytes of k per row, 3 stages
     CFG_KC_128 = 20,        // 128x128, 4 waves, 128 bytes of k per row, 2 stages
-    CFG_COUNT = 21
+    // int8 only: 256x256 direct-to-LDS tiles (one per CU for a batch of 1024
+    // 128x128 tiles), k 64, 4 stages
+    CFG_DMA_256 = 21,       // 8 waves of 128x64
+    CFG_DMA_256_W16 = 22,   // 16 waves of 64x64
+    CFG_COUNT = 23
 };
 
 const char* config_name(int cfg);

@@ -1,6 +1,6 @@
 """Autotuner for the tile configuration per size bucket.
 
-    python -m ssblasex_amd.tune [--out FILE] [--set bench|grid|all] [--quick]
+    python -m ssblasex_amd.tune [--out FILE] [--set bench|grid|large|all|grouped] [--quick]
 
 For every problem of the chosen set it times each tile configuration of the
 library (forced through SSBLAS_TILE_CONFIG) on the current GPU and records
@@ -37,7 +37,7 @@ import numpy as np
 
 CONFIGS = ["direct16", "direct32", "tile64", "tile128", "tile128x64", "tile64x128", "tile128k2", "tile128w8",
            "tile256x128", "tile64s2", "tile128s2", "tile128k2s2", "tile128w8s2", "tile256x128s2", "dma64",
-           "dma128", "dma128k2", "dma256x128", "dma128s3", "kc256x128", "kc128"]
+           "dma128", "dma128k2", "dma256x128", "dma128s3", "kc256x128", "kc128", "dma256", "dma256w16"]
 FORMS = {"i8": 0, "f32": 1, "f64": 2, "bf16x3": 3, "bf16x6": 4}
 COMPUTE = {"i8": 0, "f32": 0, "f64": 0, "bf16x3": 4, "bf16x6": 5}
 TABLE_DIR = Path(__file__).resolve().parent / "tune"
@@ -52,12 +52,15 @@ def work_class(m: int, n: int, batch: int) -> int:
     return 0 if tiles < 256 else (1 if tiles < 2048 else 2)
 
 
-def candidates(m: int, n: int):
+def candidates(form: str, m: int, n: int):
     if m <= 16 and n <= 16:
-        return ["direct16", "direct32", "tile64"]
-    if m <= 64 and n <= 64:
-        return ["direct16", "direct32", "tile64", "tile64s2", "tile64x128", "tile128x64", "dma64"]
-    return CONFIGS[2:]
+        c = ["direct16", "direct32", "tile64"]
+    elif m <= 64 and n <= 64:
+        c = ["direct16", "direct32", "tile64", "tile64s2", "tile64x128", "tile128x64", "dma64"]
+    else:
+        c = CONFIGS[2:]
+    # the dma* tiles are int8 only (other forms run tile128 under those names)
+    return c if form == "i8" else [x for x in c if not x.startswith("dma")]
 
 
 def problem_sets(name: str):
@@ -76,7 +79,15 @@ def problem_sets(name: str):
                 for d in (16, 32, 64, 128, 256, 512):
                     batch = max(8, min(4096, (1 << 27) // (d * d * d)))
                     grid.append((form, ta, tb, d, d, d, batch))
-    return {"bench": bench, "grid": grid, "all": bench + grid}[name]
+    # batches of several thousand 128x128 tiles, where the 256x256 int8 tiles
+    # (one per CU) compete with the streamed 128x128 ones
+    large = []
+    for form in ("i8", "f32"):
+        for ta in (0, 1):
+            for tb in (0, 1):
+                for d, batch in ((256, 256), (512, 64), (512, 256)):
+                    large.append((form, ta, tb, d, d, d, batch))
+    return {"bench": bench, "grid": grid, "large": large, "all": bench + grid + large}[name]
 
 
 def time_config(S, torch, form, ta, tb, m, n, k, batch, cfg, steps, warmup):
@@ -223,7 +234,7 @@ def read_table(path: Path):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--set", default="all", choices=["bench", "grid", "all", "grouped"])
+    ap.add_argument("--set", default="all", choices=["bench", "grid", "large", "all", "grouped"])
     ap.add_argument("--out", default=None)
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--steps", type=int, default=20)
@@ -237,11 +248,16 @@ def main(argv=None):
     rows = read_table(out if out.exists() else TABLE_DIR / f"{arch}.txt")
     steps = 5 if a.quick else a.steps
     t0 = time.time()
+    # about a second of work before the first timing, so that the clocks
+    # have left their idle state
+    while time.time() - t0 < 1.0:
+        time_config(S, torch, "i8", 1, 1, 512, 512, 512, 64, "tile128", 50, 0)
+    t0 = time.time()
     if a.set == "grouped":
         rows.update(tune_grouped(S, torch, ["i8", "f32", "f64"], a.quick, lambda msg: print(msg, flush=True)))
     for (form, ta, tb, m, n, k, batch) in (problem_sets(a.set) if a.set != "grouped" else []):
         res = {}
-        for cfg in candidates(m, n):
+        for cfg in candidates(form, m, n):
             try:
                 res[cfg] = time_config(S, torch, form, ta, tb, m, n, k, batch, cfg, steps, 3)
             except RuntimeError as e:
