@@ -280,6 +280,15 @@ __device__ __forceinline__ bool aligned16(const T* p)
     return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
 }
 
+// s_waitcnt vmcnt(N) leaving expcnt / lgkmcnt at their maxima (gfx9 encoding:
+// vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14]).
+template <int N>
+__device__ __forceinline__ void wait_vm()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (7 << 4) | (0xF << 8));
+}
+
 // Map a linear slot t in [0, total) to a work id so that the work ids a
 // hardware XCD sees (slots t with equal t % 8, because workgroups are
 // dispatched round-robin over the 8 XCDs) form one contiguous range:

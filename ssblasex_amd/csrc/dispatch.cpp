@@ -35,6 +35,7 @@ const char* config_name(int cfg)
     case CFG_KC_128: return "kc128";
     case CFG_DMA_256: return "dma256";
     case CFG_DMA_256_W16: return "dma256w16";
+    case CFG_DMA_256_K2: return "dma256k2";
     default: return "unknown";
     }
 }
@@ -45,7 +46,7 @@ void config_tile(int cfg, int* tm, int* tn)
                                            {64, 128},  {128, 128}, {128, 128}, {256, 128}, {64, 64},
                                            {128, 128}, {128, 128}, {128, 128}, {256, 128}, {64, 64},
                                            {128, 128}, {128, 128}, {256, 128}, {128, 128},
-                                           {256, 128}, {128, 128}, {256, 256}, {256, 256}};
+                                           {256, 128}, {128, 128}, {256, 256}, {256, 256}, {256, 256}};
     if (cfg < 0 || cfg >= CFG_COUNT) cfg = CFG_TILE_64;
     *tm = dims[cfg][0];
     *tn = dims[cfg][1];

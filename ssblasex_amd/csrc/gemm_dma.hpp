@@ -106,14 +106,6 @@ struct DmaShape {
     static_assert(S >= 2, "at least double buffered");
 };
 
-// s_waitcnt vmcnt(N) leaving expcnt / lgkmcnt at their maxima (gfx9 encoding:
-// vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14]).
-template <int N>
-__device__ __forceinline__ void wait_vm()
-{
-    static_assert(N >= 0 && N < 64, "vmcnt range");
-    __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (7 << 4) | (0xF << 8));
-}
 __device__ __forceinline__ void wait_lgkm0() { __builtin_amdgcn_s_waitcnt(0xF | (7 << 4) | (0 << 8) | (3 << 14)); }
 // A value the compiler cannot see through: address arithmetic derived from
 // it is recomputed where it is used instead of being hoisted out of the
@@ -232,14 +224,11 @@ struct DmaTile {
     {
         int32_t* C = static_cast<int32_t*>(j.C);
         const bool c_fast = j.i0 + BM <= j.m && j.j0 + BN <= j.n && ((j.ldc * 4) % 16) == 0 && aligned16(C);
-#pragma unroll
-        for (int fm = 0; fm < SH::FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < SH::FN; ++fn) {
-                const int r0 = j.i0 + wm * SH::WTM + fm * 16 + 4 * (lane >> 4);
-                const int c = j.j0 + wn * SH::WTN + fn * 16 + (lane & 15);
-                store_frag<AR>(C, j.ldc, r0, c, acc[fm][fn], j.s, j.beta0 != 0, c_fast, j.m, j.n);
-            }
+        const int rb = j.i0 + wm * SH::WTM + 4 * (lane >> 4), cb = j.j0 + wn * SH::WTN + (lane & 15);
+        store_tile<AR>(C, j.ldc, acc, j.s, j.beta0 != 0, c_fast, j.m, j.n, [&](int fm, int fn, int& r0, int& c) {
+            r0 = rb + fm * 16;
+            c = cb + fn * 16;
+        });
     }
 
     __device__ __forceinline__ static void zero(i32x4 (&acc)[SH::FM][SH::FN])

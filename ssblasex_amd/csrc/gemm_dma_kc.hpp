@@ -138,14 +138,11 @@ struct KcTile {
         Out* C = static_cast<Out*>(j.C);
         const bool c_fast = j.i0 + BM <= j.m && j.j0 + BN <= j.n && ((j.ldc * (long)sizeof(Out)) % 16) == 0 &&
                             aligned16(C);
-#pragma unroll
-        for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < FN; ++fn) {
-                const int r0 = j.i0 + wm * WTM + fm * 16 + 4 * (lane >> 4);
-                const int c = j.j0 + wn * WTN + fn * 16 + (lane & 15);
-                store_frag<AR>(C, j.ldc, r0, c, acc[fm][fn], j.s, j.beta0 != 0, c_fast, j.m, j.n);
-            }
+        const int rb = j.i0 + wm * WTM + 4 * (lane >> 4), cb = j.j0 + wn * WTN + (lane & 15);
+        store_tile<AR>(C, j.ldc, acc, j.s, j.beta0 != 0, c_fast, j.m, j.n, [&](int fm, int fn, int& r0, int& c) {
+            r0 = rb + fm * 16;
+            c = cb + fn * 16;
+        });
     }
 
     __device__ __forceinline__ static void wait_stage(int ahead)

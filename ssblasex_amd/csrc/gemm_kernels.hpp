@@ -304,6 +304,110 @@ __device__ __forceinline__ void store_frag(typename AR::Out* C, long ldc, int r0
     }
 }
 
+// 16 bytes (or two times 16) of results / old values of one fragment lane:
+// the lane's 4 consecutive rows of one column, 16-byte aligned.
+template <class Out>
+__device__ __forceinline__ void store_vec4(Out* dst, const Out (&res)[4])
+{
+    if constexpr (sizeof(Out) == 4) {
+        uint32_t w[4];
+        __builtin_memcpy(w, res, 16);
+        *gbl(reinterpret_cast<u32x4*>(dst)) = u32x4{w[0], w[1], w[2], w[3]};
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) *gbl(reinterpret_cast<f64x2*>(dst + 2 * h)) = f64x2{res[2 * h], res[2 * h + 1]};
+    }
+}
+template <class Out>
+__device__ __forceinline__ void load_vec4(const Out* src, Out (&old)[4])
+{
+    if constexpr (sizeof(Out) == 4) {
+        const u32x4 t = *gbl(reinterpret_cast<const u32x4*>(src));
+        const uint32_t w[4] = {t.x, t.y, t.z, t.w};
+        __builtin_memcpy(old, w, 16);
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const f64x2 t = *gbl(reinterpret_cast<const f64x2*>(src + 2 * h));
+            old[2 * h] = t[0];
+            old[2 * h + 1] = t[1];
+        }
+    }
+}
+
+// Fused alpha/beta epilogue of a wave's FM x FN fragments; pos(fm, fn, r0, c)
+// gives the lane's first row and its column in fragment (fm, fn).  The
+// beta == 0 / beta != 0 and whole / edge choices are made once for the wave,
+// outside the fragment loops: a load issued conditionally inside the loop
+// puts a vmcnt wait in front of every store, and on gfx9 that wait also
+// waits for every store issued before it, so the tile's stores would leave
+// one round trip at a time.  With beta != 0 the old values are read D
+// fragments ahead through a ring of D fragments: fragment f + D is requested
+// once f's results are computed and before they are stored, so the wait for
+// an old value never covers a store.  fp64 results keep the per-fragment
+// routine: the split epilogue made the fp64 128x128 tile (at the edge of
+// its register budget) 20 % slower, for no gain in its stores.
+template <class AR, int FM, int FN, class Pos>
+__device__ __forceinline__ void store_tile(typename AR::Out* C, long ldc, const typename AR::Acc (&acc)[FM][FN],
+                                           const Scalars& sc, bool beta0, bool fast, int m, int n, Pos pos)
+{
+    using Out = typename AR::Out;
+    constexpr int F = FM * FN;
+    if constexpr (sizeof(Out) == 8) {
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn) {
+                int r0, c;
+                pos(fm, fn, r0, c);
+                store_frag<AR>(C, ldc, r0, c, acc[fm][fn], sc, beta0, fast, m, n);
+            }
+        return;
+    }
+    if (fast && beta0) {
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn) {
+                int r0, c;
+                pos(fm, fn, r0, c);
+                Out res[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) res[v] = epi((Out)acc[fm][fn][v], Out(0), sc, true);
+                store_vec4<Out>(C + (long)c * ldc + r0, res);
+            }
+    } else if (fast) {
+        constexpr int D = F < 4 ? F : 4;
+        Out old[D][4];
+        auto load = [&](int f) {
+            int r0, c;
+            pos(f / FN, f % FN, r0, c);
+            load_vec4<Out>(C + (long)c * ldc + r0, old[f % D]);
+        };
+#pragma unroll
+        for (int f = 0; f < D; ++f) load(f);
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+            int r0, c;
+            pos(f / FN, f % FN, r0, c);
+            Out res[4];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) res[v] = epi((Out)acc[f / FN][f % FN][v], old[f % D][v], sc, false);
+            if (f + D < F) load(f + D);
+            store_vec4<Out>(C + (long)c * ldc + r0, res);
+        }
+    } else {
+#pragma unroll
+        for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn) {
+                int r0, c;
+                pos(fm, fn, r0, c);
+                store_frag<AR>(C, ldc, r0, c, acc[fm][fn], sc, beta0, false, m, n);
+            }
+    }
+}
+
 // The old C values of one fragment (the lane's 4 rows of column c), read
 // before the k-loop of a small tile when beta != 0, so that their latency
 // overlaps the operand loads instead of following the last MFMA.  Zero
@@ -541,20 +645,22 @@ __device__ __forceinline__ void tile_job(const TileJob& j, char* smem)
     }
 
     // fused alpha/beta epilogue
+    const int rb = j.i0 + wm * S::WTM + 4 * (lane >> 4), cb = j.j0 + wn * S::WTN + (lane & 15);
+    if constexpr (PRE) {
+        if (j.beta0 == 0) {
 #pragma unroll
-    for (int fm = 0; fm < S::FM; ++fm)
+            for (int fm = 0; fm < S::FM; ++fm)
 #pragma unroll
-        for (int fn = 0; fn < S::FN; ++fn) {
-            const int r0 = j.i0 + wm * S::WTM + fm * 16 + 4 * (lane >> 4);
-            const int c = j.j0 + wn * S::WTN + fn * 16 + (lane & 15);
-            if constexpr (PRE) {
-                if (j.beta0 == 0) {
-                    store_frag_old<AR>(C, j.ldc, r0, c, acc[fm][fn], j.s, c_fast, j.m, j.n, old[fm][fn]);
-                    continue;
-                }
-            }
-            store_frag<AR>(C, j.ldc, r0, c, acc[fm][fn], j.s, j.beta0 != 0, c_fast, j.m, j.n);
+                for (int fn = 0; fn < S::FN; ++fn)
+                    store_frag_old<AR>(C, j.ldc, rb + fm * 16, cb + fn * 16, acc[fm][fn], j.s, c_fast, j.m, j.n,
+                                       old[fm][fn]);
+            return;
         }
+    }
+    store_tile<AR>(C, j.ldc, acc, j.s, j.beta0 != 0, c_fast, j.m, j.n, [&](int fm, int fn, int& r0, int& c) {
+        r0 = rb + fm * 16;
+        c = cb + fn * 16;
+    });
 }
 
 __device__ __forceinline__ TileJob make_job(const KParams& p, int inst, int i0, int j0)
@@ -686,16 +792,20 @@ __device__ __forceinline__ void direct_job(const TileJob& j)
             for (int fn = 0; fn < FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
     }
 
+    const int rb = j.i0 + 4 * (lane >> 4), cb = j.j0 + (lane & 15);
+    if (j.beta0 == 0) {
 #pragma unroll
-    for (int fm = 0; fm < FM; ++fm)
+        for (int fm = 0; fm < FM; ++fm)
 #pragma unroll
-        for (int fn = 0; fn < FN; ++fn)
-            if (j.beta0 == 0)
-                store_frag_old<AR>(C, j.ldc, j.i0 + fm * 16 + 4 * (lane >> 4), j.j0 + fn * 16 + (lane & 15),
-                                   acc[fm][fn], j.s, c_fast, j.m, j.n, old[fm][fn]);
-            else
-                store_frag<AR>(C, j.ldc, j.i0 + fm * 16 + 4 * (lane >> 4), j.j0 + fn * 16 + (lane & 15), acc[fm][fn],
-                           j.s, j.beta0 != 0, c_fast, j.m, j.n);
+            for (int fn = 0; fn < FN; ++fn)
+                store_frag_old<AR>(C, j.ldc, rb + fm * 16, cb + fn * 16, acc[fm][fn], j.s, c_fast, j.m, j.n,
+                                   old[fm][fn]);
+    } else {
+        store_tile<AR>(C, j.ldc, acc, j.s, true, c_fast, j.m, j.n, [&](int fm, int fn, int& r0, int& c) {
+            r0 = rb + fm * 16;
+            c = cb + fn * 16;
+        });
+    }
 }
 
 template <class AR, bool AKC, bool BKC, int WTM, int WTN>

@@ -48,7 +48,8 @@ enum TileConfig : int {
     // 128x128 tiles), k 64, 4 stages
     CFG_DMA_256 = 21,       // 8 waves of 128x64
     CFG_DMA_256_W16 = 22,   // 16 waves of 64x64
-    CFG_COUNT = 23
+    CFG_DMA_256_K2 = 23,    // 8 waves of 128x64, k 128, 2 stages
+    CFG_COUNT = 24
 };
 
 const char* config_name(int cfg);

@@ -197,6 +197,7 @@ hipError_t run_cfg(const KParams& p, int cfg, hipStream_t st)
     case CFG_KC_128: return run_kc<AR, AKC, BKC, 128, 128, 2, 2, 2, CFG_TILE_128, 2>(p, st);
     case CFG_DMA_256: return run_dma<AR, AKC, BKC, 256, 256, 64, 2, 4, 4>(p, st);
     case CFG_DMA_256_W16: return run_dma<AR, AKC, BKC, 256, 256, 64, 4, 4, 4>(p, st);
+    case CFG_DMA_256_K2: return run_dma<AR, AKC, BKC, 256, 256, 128, 2, 4, 2>(p, st);
     default: return hipErrorInvalidValue;
     }
 }
