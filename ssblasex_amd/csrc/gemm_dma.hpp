@@ -219,16 +219,18 @@ struct DmaTile {
         }
     }
 
-    __device__ __forceinline__ static void epilogue(const TileJob& j, i32x4 (&acc)[SH::FM][SH::FN], int wm, int wn,
-                                                    int lane)
+    // Returns store_tile's count of vector memory instructions (-1: not fixed).
+    __device__ __forceinline__ static int epilogue(const TileJob& j, i32x4 (&acc)[SH::FM][SH::FN], int wm, int wn,
+                                                   int lane)
     {
         int32_t* C = static_cast<int32_t*>(j.C);
         const bool c_fast = j.i0 + BM <= j.m && j.j0 + BN <= j.n && ((j.ldc * 4) % 16) == 0 && aligned16(C);
         const int rb = j.i0 + wm * SH::WTM + 4 * (lane >> 4), cb = j.j0 + wn * SH::WTN + (lane & 15);
-        store_tile<AR>(C, j.ldc, acc, j.s, j.beta0 != 0, c_fast, j.m, j.n, [&](int fm, int fn, int& r0, int& c) {
-            r0 = rb + fm * 16;
-            c = cb + fn * 16;
-        });
+        return store_tile<AR>(C, j.ldc, acc, j.s, j.beta0 != 0, c_fast, j.m, j.n,
+                              [&](int fm, int fn, int& r0, int& c) {
+                                  r0 = rb + fm * 16;
+                                  c = cb + fn * 16;
+                              });
     }
 
     __device__ __forceinline__ static void zero(i32x4 (&acc)[SH::FM][SH::FN])
@@ -241,6 +243,37 @@ struct DmaTile {
 
     // Wait until at most `ahead` stages beyond the one about to be used are
     // still in flight (DMA), then barrier.
+    // Streaming wait for one k-block: at most `ahead` later k-blocks and, when
+    // a tile's epilogue was issued after the block, that epilogue's vector
+    // memory instructions (ecase 1: F = FM * FN, ecase 2: 2F) may still be in
+    // flight; then barrier.  gfx9 retires a wave's loads and stores in
+    // order, so without the epilogue term the wait would also wait for the
+    // stores of the tile before.  Counts above the 6-bit field wait at 63
+    // (waiting longer is always safe).
+    template <int A = S - 2>
+    __device__ __forceinline__ static void wait_ring(int ahead, int ecase)
+    {
+        constexpr int F = SH::FM * SH::FN;
+        constexpr int N0 = A * SH::PER_WAVE;
+        constexpr int N1 = N0 + F < 63 ? N0 + F : 63;
+        constexpr int N2 = N0 + 2 * F < 63 ? N0 + 2 * F : 63;
+        if constexpr (A == 0) {
+            if (ecase == 2) wait_vm<N2>();
+            else if (ecase == 1) wait_vm<N1>();
+            else wait_vm<0>();
+        } else {
+            if (ahead == A) {
+                if (ecase == 2) wait_vm<N2>();
+                else if (ecase == 1) wait_vm<N1>();
+                else wait_vm<N0>();
+            } else {
+                wait_ring<A - 1>(ahead, ecase);
+                return;
+            }
+        }
+        lds_barrier();
+    }
+
     __device__ __forceinline__ static void wait_stage(int ahead)
     {
         if (ahead >= S - 2) wait_vm<(S - 2) * SH::PER_WAVE>();
@@ -352,6 +385,9 @@ dma_stream_gemm_i8(const KParams p)
         return true;
     };
 
+    constexpr int F = SH::FM * SH::FN;
+    long u_e = 0;   // k-blocks u < u_e were issued before the last epilogue's memory instructions
+    int ecase = 0;  // that epilogue: 0 drained, 1 F instructions, 2 2F instructions
     i32x4 acc[SH::FM][SH::FN];
     while (t_c < p.total) {
         if (!T::eligible(cur)) {
@@ -366,6 +402,8 @@ dma_stream_gemm_i8(const KParams p)
             ij = cur;
             i_ok = T::eligible(ij);
             u_c = u_i = 0;
+            u_e = 0;
+            ecase = 0;
             lds_barrier();
             continue;
         }
@@ -373,13 +411,24 @@ dma_stream_gemm_i8(const KParams p)
         }
         T::zero(acc);
         for (int kt = 0; kt < nk; ++kt) {
-            T::wait_stage((int)min((long)(S - 2), u_i - u_c - 1));
+            T::wait_ring((int)min((long)(S - 2), u_i - u_c - 1), u_c < u_e ? ecase : 0);
             while (u_i < u_c + S && issue_next()) {
             }
             T::compute(acc, smem + (u_c % S) * SH::STAGE, wm, wn, lane);
             ++u_c;
         }
-        T::epilogue(cur, acc, wm, wn, lane);
+        __asm__ __volatile__("" ::: "memory");  // the k-block loads above stay in front of the epilogue
+        const int ne = T::epilogue(cur, acc, wm, wn, lane);
+        __asm__ __volatile__("" ::: "memory");
+        if (ne == F) {
+            ecase = 1;
+        } else if (ne == 2 * F) {
+            ecase = 2;
+        } else {
+            wait_vm<0>();
+            ecase = 0;
+        }
+        u_e = u_i;
         t_c += G;
         if (t_c < p.total) cur = job_of(t_c);
     }

@@ -347,8 +347,11 @@ __device__ __forceinline__ void load_vec4(const Out* src, Out (&old)[4])
 // an old value never covers a store.  fp64 results keep the per-fragment
 // routine: the split epilogue made the fp64 128x128 tile (at the edge of
 // its register budget) 20 % slower, for no gain in its stores.
+// Returns the number of vector memory instructions the wave issued (F
+// stores for beta == 0, F loads and F stores for beta != 0, F = FM * FN) on
+// the two whole-tile paths, -1 on the others (count not fixed).
 template <class AR, int FM, int FN, class Pos>
-__device__ __forceinline__ void store_tile(typename AR::Out* C, long ldc, const typename AR::Acc (&acc)[FM][FN],
+__device__ __forceinline__ int store_tile(typename AR::Out* C, long ldc, const typename AR::Acc (&acc)[FM][FN],
                                            const Scalars& sc, bool beta0, bool fast, int m, int n, Pos pos)
 {
     using Out = typename AR::Out;
@@ -362,26 +365,30 @@ __device__ __forceinline__ void store_tile(typename AR::Out* C, long ldc, const 
                 pos(fm, fn, r0, c);
                 store_frag<AR>(C, ldc, r0, c, acc[fm][fn], sc, beta0, fast, m, n);
             }
-        return;
+        return -1;
     }
+    // Visit order: with FM even, fragment rows 2q and 2q + 1 of one column
+    // block go out back to back, so the two 64-byte halves that a column's
+    // lanes write of one 128-byte line leave together.
+    auto fm_of = [](int f) { return FM % 2 ? f / FN : 2 * (f / (2 * FN)) + (f % 2); };
+    auto fn_of = [](int f) { return FM % 2 ? f % FN : (f % (2 * FN)) / 2; };
     if (fast && beta0) {
 #pragma unroll
-        for (int fm = 0; fm < FM; ++fm)
+        for (int f = 0; f < F; ++f) {
+            int r0, c;
+            pos(fm_of(f), fn_of(f), r0, c);
+            Out res[4];
 #pragma unroll
-            for (int fn = 0; fn < FN; ++fn) {
-                int r0, c;
-                pos(fm, fn, r0, c);
-                Out res[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) res[v] = epi((Out)acc[fm][fn][v], Out(0), sc, true);
-                store_vec4<Out>(C + (long)c * ldc + r0, res);
-            }
+            for (int v = 0; v < 4; ++v) res[v] = epi((Out)acc[fm_of(f)][fn_of(f)][v], Out(0), sc, true);
+            store_vec4<Out>(C + (long)c * ldc + r0, res);
+        }
+        return F;
     } else if (fast) {
         constexpr int D = F < 4 ? F : 4;
         Out old[D][4];
         auto load = [&](int f) {
             int r0, c;
-            pos(f / FN, f % FN, r0, c);
+            pos(fm_of(f), fn_of(f), r0, c);
             load_vec4<Out>(C + (long)c * ldc + r0, old[f % D]);
         };
 #pragma unroll
@@ -389,13 +396,15 @@ __device__ __forceinline__ void store_tile(typename AR::Out* C, long ldc, const 
 #pragma unroll
         for (int f = 0; f < F; ++f) {
             int r0, c;
-            pos(f / FN, f % FN, r0, c);
+            pos(fm_of(f), fn_of(f), r0, c);
             Out res[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) res[v] = epi((Out)acc[f / FN][f % FN][v], old[f % D][v], sc, false);
+            for (int v = 0; v < 4; ++v)
+                res[v] = epi((Out)acc[fm_of(f)][fn_of(f)][v], old[f % D][v], sc, false);
             if (f + D < F) load(f + D);
             store_vec4<Out>(C + (long)c * ldc + r0, res);
         }
+        return 2 * F;
     } else {
 #pragma unroll
         for (int fm = 0; fm < FM; ++fm)
@@ -405,6 +414,7 @@ __device__ __forceinline__ void store_tile(typename AR::Out* C, long ldc, const 
                 pos(fm, fn, r0, c);
                 store_frag<AR>(C, ldc, r0, c, acc[fm][fn], sc, beta0, false, m, n);
             }
+        return -1;
     }
 }
 

@@ -115,7 +115,7 @@ def test_int8_dma_tiles(cfg, ta, tb):
     check(make_batch("i8", ta, tb, 256, 320, 64, 3, alpha=-1, beta=2, seed=65))
 
 
-@pytest.mark.parametrize("cfg", ["dma64", "dma128", "dma128s3", "dma256x128"])
+@pytest.mark.parametrize("cfg", ["dma64", "dma128", "dma128s3", "dma256x128", "dma256", "dma256k2"])
 @pytest.mark.parametrize("grid", ["3", "8"])
 @pytest.mark.parametrize("ta,tb", OPS)
 def test_int8_stream_several_tiles_per_workgroup(cfg, grid, ta, tb):
@@ -134,6 +134,11 @@ def test_int8_stream_several_tiles_per_workgroup(cfg, grid, ta, tb):
     try:
         check(make_batch("i8", ta, tb, 256, 384, 128, 5, alpha=3, beta=0, seed=66))
         check(make_batch("i8", ta, tb, 256, 256, 192, 4, alpha=-1, beta=1, seed=67))
+        # four 128-k blocks per tile: the next tile's blocks are requested
+        # before a tile's C is written (beta 0: stores only; beta 1: loads
+        # of the old C too), and waited for past that epilogue
+        check(make_batch("i8", ta, tb, 256, 512, 512, 3, alpha=1, beta=0, seed=69))
+        check(make_batch("i8", ta, tb, 512, 256, 512, 3, alpha=-3, beta=1, seed=70))
         b = make_batch("i8", ta, tb, 256, 256, 256, 5, alpha=2, beta=0, seed=68)
         d = DeviceBatch(b)
         x = d.A[2]
