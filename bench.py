@@ -50,7 +50,8 @@ def parse():
                     help="only time a device-to-device copy of MIB MiB (HBM calibration for the counters)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--case", default=None, metavar="ID:FORM",
-                    help="time only one other configuration, e.g. c4:bf16x6, c3:ozaki_s9, c5:all (profiling runs)")
+                    help="time only one other configuration, e.g. c4:bf16x6, c3:ozaki_s9, c5:all (c5 also takes small, "
+                         "large, fp32, int8; profiling runs)")
     return ap.parse_args()
 
 
@@ -319,9 +320,12 @@ def grouped_c5(args, stream, only=None):
     small = [i for i in range(count) if max(inst[i][3:6]) <= 32]
     subsets = {"all": ("all", range(count)), "small": ("small (max dim <= 32)", small),
                "large": ("large (min dim >= 256)", big)}
+    # only on request (profiling runs): the fp32 and the int8 instances on their own
+    extra = {"fp32": ("fp32 instances", [i for i in range(count) if inst[i][0] == "f32"]),
+             "int8": ("int8 instances", [i for i in range(count) if inst[i][0] == "i8"])}
     out = {}
-    for key, (label, sel) in subsets.items():
-        if only is None or only == key:
+    for key, (label, sel) in list(subsets.items()) + list(extra.items()):
+        if (only is None and key in subsets) or only == key:
             out[label] = run(sel)
     del bufs
     torch.cuda.empty_cache()
