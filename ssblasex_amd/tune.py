@@ -87,6 +87,9 @@ def problem_sets(name: str):
             for tb in (0, 1):
                 for d, batch in ((256, 256), (512, 64), (512, 256)):
                     large.append((form, ta, tb, d, d, d, batch))
+    for ta in (0, 1):
+        for tb in (0, 1):
+            large.append(("f64", ta, tb, 256, 256, 256, 512))
     return {"bench": bench, "grid": grid, "large": large, "all": bench + grid + large}[name]
 
 
