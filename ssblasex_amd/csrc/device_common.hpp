@@ -280,6 +280,15 @@ __device__ __forceinline__ bool aligned16(const T* p)
     return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
 }
 
+// Buffer descriptor word 3 for gfx9-family targets: 32-bit data format,
+// raw (untyped) addressing.
+constexpr int GFX9_BUFFER_RSRC_WORD3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, GFX9_BUFFER_RSRC_WORD3);
+}
+
 // s_waitcnt vmcnt(N) leaving expcnt / lgkmcnt at their maxima (gfx9 encoding:
 // vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14]).
 template <int N>

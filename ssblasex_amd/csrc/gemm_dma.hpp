@@ -220,17 +220,19 @@ struct DmaTile {
     }
 
     // Returns store_tile's count of vector memory instructions (-1: not fixed).
+    // last: the workgroup's last tile (its stores are written through L2).
     __device__ __forceinline__ static int epilogue(const TileJob& j, i32x4 (&acc)[SH::FM][SH::FN], int wm, int wn,
-                                                   int lane)
+                                                   int lane, bool last)
     {
         int32_t* C = static_cast<int32_t*>(j.C);
         const bool c_fast = j.i0 + BM <= j.m && j.j0 + BN <= j.n && ((j.ldc * 4) % 16) == 0 && aligned16(C);
         const int rb = j.i0 + wm * SH::WTM + 4 * (lane >> 4), cb = j.j0 + wn * SH::WTN + (lane & 15);
-        return store_tile<AR>(C, j.ldc, acc, j.s, j.beta0 != 0, c_fast, j.m, j.n,
+        return store_tile<AR, SH::FM, SH::FN, true>(C, j.ldc, acc, j.s, j.beta0 != 0, c_fast, j.m, j.n,
                               [&](int fm, int fn, int& r0, int& c) {
                                   r0 = rb + fm * 16;
                                   c = cb + fn * 16;
-                              });
+                              },
+                              last);
     }
 
     __device__ __forceinline__ static void zero(i32x4 (&acc)[SH::FM][SH::FN])
@@ -285,7 +287,7 @@ struct DmaTile {
 // One tile, S-stage pipeline inside the tile (the previous tile's buffers
 // must be free: the routine starts with a barrier).
 template <bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int S>
-__device__ __forceinline__ void dma_tile_job(const TileJob& j, char* smem)
+__device__ __forceinline__ void dma_tile_job(const TileJob& j, char* smem, bool last)
 {
     using T = DmaTile<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>;
     using SH = typename T::SH;
@@ -317,7 +319,7 @@ __device__ __forceinline__ void dma_tile_job(const TileJob& j, char* smem)
         }
         T::compute(acc, smem + (kt % S) * SH::STAGE, wm, wn, lane);
     }
-    T::epilogue(j, acc, wm, wn, lane);
+    T::epilogue(j, acc, wm, wn, lane, last);
 }
 
 template <bool AKC, bool BKC, int BM, int BN, int BK, int WAVES_M, int WAVES_N, int S>
@@ -332,7 +334,7 @@ dma_gemm_i8(const KParams p)
         const int inst = (int)(w / tiles_per_inst);
         const int rem = (int)(w % tiles_per_inst);
         const TileJob j = make_job(p, inst, (rem % p.tiles_m) * BM, (rem / p.tiles_m) * BN);
-        dma_tile_job<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>(j, smem);
+        dma_tile_job<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>(j, smem, t + gridDim.x >= p.total);
     }
 }
 
@@ -393,7 +395,7 @@ dma_stream_gemm_i8(const KParams p)
         if (!T::eligible(cur)) {
             // nothing of this tile was issued (the stream stops in front of it)
             wait_vm<0>();
-            dma_tile_job<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>(cur, smem);
+            dma_tile_job<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>(cur, smem, t_c + G >= p.total);
             t_c += G;
             if (t_c >= p.total) break;
             cur = job_of(t_c);
@@ -418,7 +420,7 @@ dma_stream_gemm_i8(const KParams p)
             ++u_c;
         }
         __asm__ __volatile__("" ::: "memory");  // the k-block loads above stay in front of the epilogue
-        const int ne = T::epilogue(cur, acc, wm, wn, lane);
+        const int ne = T::epilogue(cur, acc, wm, wn, lane, t_c + G >= p.total);
         __asm__ __volatile__("" ::: "memory");
         if (ne == F) {
             ecase = 1;

@@ -27,15 +27,6 @@
 
 namespace ssb {
 
-// Buffer descriptor word 3 for gfx9-family targets: 32-bit data format,
-// raw (untyped) addressing.
-constexpr int GFX9_BUFFER_RSRC_WORD3 = 0x00020000;
-
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, GFX9_BUFFER_RSRC_WORD3);
-}
-
 // Elements from index `nvalid` on of a word (WB bytes, E-byte elements) set to 0.
 template <int WB, int E>
 __device__ __forceinline__ void clear_tail(uint32_t* d, int nvalid)
