@@ -291,7 +291,7 @@ __device__ __forceinline__ void dma_tile_job(const TileJob& j, char* smem, bool 
 {
     using T = DmaTile<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>;
     using SH = typename T::SH;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave % WAVES_M, wn = wave / WAVES_M;
     const int nk = (j.k + BK - 1) / BK;
     const bool dma = T::eligible(j);
@@ -351,7 +351,7 @@ dma_stream_gemm_i8(const KParams p)
     using T = DmaTile<AKC, BKC, BM, BN, BK, WAVES_M, WAVES_N, S>;
     using SH = typename T::SH;
     __shared__ __attribute__((aligned(1024))) char smem[SH::LDS_BYTES];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave % WAVES_M, wn = wave / WAVES_M;
     const int tiles_per_inst = p.tiles_m * p.tiles_n;
     const int nk = p.k / BK;
