@@ -1,31 +1,45 @@
 """Benchmark of ssblasGemmBatchedEx on one MI355X.
 
-Default workload (the headline configuration): int8 -> int32, transa = transb
-= T, m = n = k = 512, batchCount = 64, lda = ldb = ldc = 512, A/B uniform
-integers in [-128, 127], C uniform in [-1000, 1000], alpha = 1, beta = 0;
-every A[i] / B[i] / C[i] its own allocation, pointer arrays device resident.
+Headline workload: int8 -> int32, transa = transb = T, m = n = k = 512,
+batchCount = 64, lda = ldb = ldc = 512, A/B uniform integers in [-128, 127],
+C uniform in [-1000, 1000], alpha = 1, beta = 0; every A[i] / B[i] / C[i] its
+own allocation, pointer arrays device resident.  `value` is its int8 TOPS.
 
 A step is one ssblasGemmBatchedExAsync call over the whole batch with the
 inputs already in HBM.  W untimed warmup steps, then K steps bracketed by
 HIP events on the launch stream and device synchronisation.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--beta 0|1]
-                    [--no-cpu-baseline] [--secondary] [--grouped]
+The same line carries, under "split_vs_native", the metric's second half on
+the other benchmark configurations, each timed the same way (its own warm-up,
+then max(20, K/10) steps) with the normwise error of one call on a seeded
+sample of instances against the double-double FP64 oracle:
+  * fp32 TN m=n=k=256, batchCount 4096, N(0,1), alpha 1.5, beta -0.5:
+    native fp32 vs BF16x3 vs BF16x6;
+  * fp64 NT m=n=k=32, batchCount 65536, U[-1,1]*2^U{-8..8}, beta 0:
+    native fp64 vs Ozaki (9 slices, and the automatic slice count);
+  * fp32 NN m=n=k=64, batchCount 128, alpha 1, beta 0.5 (separate allocations);
+  * the headline batch with beta = 1.
+The CPU leg (rank 0 only) times the reference example's baselines on the
+host cores -- its OpenMP int8 loop and its per-instance cblas_?gemm loop
+(scipy's OpenBLAS) -- and evaluates the errors above with the oracle as the
+checker.
 
-Prints one JSON line.  --secondary adds the other benchmark configurations
-(fp32 64^3 x128 NN, fp64 32^3 x65536 NT native vs Ozaki, fp64 256^3 x512,
-fp32 256^3 x4096 TN native vs BF16x3/x6 with ld 256 and 264, the headline
-batch with beta = 1) under "secondary"; --grouped (implied by
---secondary) adds the grouped 10000-instance uneven batch under "grouped_c5".
---case ID:FORM times only one of those (c1, c3, c3l, c4, c4p, c2b1 with a
-form name such as native, bf16x6 or ozaki_s9; c5 with all, small or large)
-and prints its own JSON line, for profiler runs.
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--beta 0|1]
+                    [--no-cpu-baseline] [--no-split] [--secondary] [--grouped]
+
+--secondary adds more configurations (fp64 NN 256^3 x512 native vs Ozaki,
+the fp32 TN batch at lda=ldb=ldc=264) and implies --grouped, which adds the
+grouped 10000-instance uneven batch under "grouped_c5".  --case ID:FORM times
+only one configuration (c1, c3, c3l, c4, c4p, c2b1 with a form name such as
+native, bf16x6, ozaki_s9 or ozaki_auto; c5 with all, small, large, fp32 or
+int8) and prints its own JSON line, for profiler runs.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -35,6 +49,7 @@ METRIC = "batched GEMM TFLOP/s|TOPS & % MFMA peak; split-precision vs native at 
 PEAK = {"int8": 5033.2, "f32": 157.3, "f64": 78.6, "bf16": 2516.6}  # dense, TFLOP/s | TOPS
 HBM_PEAK_GBS = 8000.0
 SEED = 0x55B1A5
+ROOT = os.path.dirname(os.path.abspath(__file__))
 
 
 def parse():
@@ -44,6 +59,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2000)
     ap.add_argument("--beta", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-split", action="store_true", help="leave out the split_vs_native block")
     ap.add_argument("--secondary", action="store_true")
     ap.add_argument("--grouped", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -53,10 +69,6 @@ def parse():
                     help="time only one other configuration, e.g. c4:bf16x6, c3:ozaki_s9, c5:all (c5 also takes small, "
                          "large, fp32, int8; profiling runs)")
     return ap.parse_args()
-
-
-def gen_i8(rng, n):
-    return rng.integers(-128, 128, size=n, dtype=np.int64).astype(np.int8)
 
 
 def time_steps(fn, steps, warmup, stream):
@@ -75,66 +87,126 @@ def time_steps(fn, steps, warmup, stream):
     return start.elapsed_time(end) / steps  # ms per step
 
 
-class Problem:
-    """One batched GEMM with separately allocated instances on the device."""
+def warm_for(fn, seconds, stream):
+    """Untimed calls for about `seconds` of device time (the clocks leave
+    idle before a configuration's timed steps)."""
+    import torch
 
-    def __init__(self, t, ta, tb, m, n, k, batch, alpha, beta, seed, dist="uniform", pad=0):
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        fn()
+        n += 1
+        if n % 8 == 0:
+            torch.cuda.synchronize()
+            if time.perf_counter() - t0 >= seconds:
+                return
+
+
+TYPECODES = {"i8": (0, 3), "f32": (4, 4), "f64": (5, 5)}  # (A/B type, C type) codes of the C ABI
+
+
+class Problem:
+    """One batched GEMM on the device.
+
+    separate=True: every A[i] / B[i] / C[i] its own allocation (the
+    reference's callers' layout), drawn with numpy from `seed`.
+    separate=False: one allocation per operand, instance i at a fixed
+    stride, drawn on the device from `seed` (a handful of dispatches however
+    large the batch).  Either way the pointer arrays are device resident."""
+
+    def __init__(self, t, ta, tb, m, n, k, batch, alpha, beta, seed, dist="uniform", pad=0, separate=True):
         import torch
+
         import ssblasex_amd as S
 
         self.t, self.ta, self.tb, self.m, self.n, self.k, self.batch = t, ta, tb, m, n, k, batch
         self.alpha, self.beta = alpha, beta
         ra, ca = (m, k) if ta == 0 else (k, m)
         rb, cb = (k, n) if tb == 0 else (n, k)
+        self.ra, self.ca, self.rb, self.cb = ra, ca, rb, cb
         self.lda, self.ldb, self.ldc = ra + pad, rb + pad, m + pad
-        rng = np.random.default_rng(seed)
-        code = {"i8": (S.DataType.R_8I, S.DataType.R_32I), "f32": (S.DataType.R_32F, S.DataType.R_32F),
-                "f64": (S.DataType.R_64F, S.DataType.R_64F)}[t]
-        self.ab, self.cc = code
-        tdt = {"i8": (torch.int8, torch.int32), "f32": (torch.float32, torch.float32),
-               "f64": (torch.float64, torch.float64)}[t]
-        dev = "cuda"
-        self.A, self.B, self.C = [], [], []
-        self.hostA, self.hostB, self.hostC = [], [], []
-        gen_chunk = max(1, min(batch, (1 << 24) // max(1, self.lda * ca)))
-        for i0 in range(0, batch, gen_chunk):
-            nb = min(gen_chunk, batch - i0)
-            if t == "i8":
-                a = gen_i8(rng, nb * self.lda * ca)
-                b = gen_i8(rng, nb * self.ldb * cb)
-                c = rng.integers(-1000, 1001, size=nb * self.ldc * n, dtype=np.int64).astype(np.int32)
-            else:
-                npdt = np.float32 if t == "f32" else np.float64
+        self.ab, self.cc = TYPECODES[t]
+        self.sa, self.sb, self.sc = self.lda * ca, self.ldb * cb, self.ldc * n  # elements per instance
+        if separate:
+            rng = np.random.default_rng(seed)
+            npdt = {"i8": np.int8, "f32": np.float32, "f64": np.float64}[t]
+
+            def draw(sz, c_operand):
+                if t == "i8":
+                    if c_operand:
+                        return rng.integers(-1000, 1001, size=sz, dtype=np.int32)
+                    return rng.integers(-128, 128, size=sz, dtype=np.int8)
                 if dist == "normal":
-                    a = rng.standard_normal(nb * self.lda * ca).astype(npdt)
-                    b = rng.standard_normal(nb * self.ldb * cb).astype(npdt)
-                    c = rng.standard_normal(nb * self.ldc * n).astype(npdt)
-                elif dist == "wide":
-                    def w(sz):
-                        return (rng.uniform(-1, 1, sz) * np.exp2(rng.integers(-8, 9, sz))).astype(npdt)
-                    a, b, c = w(nb * self.lda * ca), w(nb * self.ldb * cb), w(nb * self.ldc * n)
-                else:
-                    a = rng.uniform(-1, 1, nb * self.lda * ca).astype(npdt)
-                    b = rng.uniform(-1, 1, nb * self.ldb * cb).astype(npdt)
-                    c = rng.uniform(-1, 1, nb * self.ldc * n).astype(npdt)
-            ta_ = torch.from_numpy(a).to(dev)
-            tb_ = torch.from_numpy(b).to(dev)
-            tc_ = torch.from_numpy(c).to(dev)
-            for j in range(nb):
-                # separate allocations, as the reference's callers make them
-                self.A.append(ta_[j * self.lda * ca:(j + 1) * self.lda * ca].clone())
-                self.B.append(tb_[j * self.ldb * cb:(j + 1) * self.ldb * cb].clone())
-                self.C.append(tc_[j * self.ldc * n:(j + 1) * self.ldc * n].clone())
-            if len(self.hostA) < 64:
-                for j in range(min(nb, 64 - len(self.hostA))):
-                    self.hostA.append(a[j * self.lda * ca:(j + 1) * self.lda * ca])
-                    self.hostB.append(b[j * self.ldb * cb:(j + 1) * self.ldb * cb])
-                    self.hostC.append(c[j * self.ldc * n:(j + 1) * self.ldc * n])
-            del ta_, tb_, tc_
-        self.pA = S.pointer_array(self.A)
-        self.pB = S.pointer_array(self.B)
-        self.pC = S.pointer_array(self.C)
+                    return rng.standard_normal(sz).astype(npdt)
+                if dist == "wide":
+                    return (rng.uniform(-1, 1, sz) * np.exp2(rng.integers(-8, 9, sz))).astype(npdt)
+                return rng.uniform(-1, 1, sz).astype(npdt)
+
+            a = torch.from_numpy(draw(batch * self.sa, False)).to("cuda")
+            b = torch.from_numpy(draw(batch * self.sb, False)).to("cuda")
+            c = torch.from_numpy(draw(batch * self.sc, True)).to("cuda")
+            # separate allocations, as the reference's callers make them
+            self.A = [a[i * self.sa:(i + 1) * self.sa].clone() for i in range(batch)]
+            self.B = [b[i * self.sb:(i + 1) * self.sb].clone() for i in range(batch)]
+            self.C = [c[i * self.sc:(i + 1) * self.sc].clone() for i in range(batch)]
+            del a, b, c
+            self.pA, self.pB, self.pC = S.pointer_array(self.A), S.pointer_array(self.B), S.pointer_array(self.C)
+        else:
+            gen = torch.Generator(device="cuda").manual_seed(seed)
+            tdt = {"i8": torch.int8, "f32": torch.float32, "f64": torch.float64}[t]
+
+            def draw(shape, c_operand):
+                if t == "i8":
+                    if c_operand:
+                        return torch.randint(-1000, 1001, shape, device="cuda", dtype=torch.int32, generator=gen)
+                    return torch.randint(-128, 128, shape, device="cuda", dtype=torch.int8, generator=gen)
+                if dist == "normal":
+                    return torch.randn(shape, device="cuda", dtype=tdt, generator=gen)
+                u = torch.rand(shape, device="cuda", dtype=tdt, generator=gen) * 2 - 1
+                if dist == "wide":
+                    e = torch.randint(-8, 9, shape, device="cuda", generator=gen).to(tdt)
+                    return u * torch.exp2(e)
+                return u
+
+            self.Aflat = draw((batch, self.sa), False)
+            self.Bflat = draw((batch, self.sb), False)
+            self.Cflat = draw((batch, self.sc), True)
+            self.A = self.B = self.C = None
+            self.pA = S.pointer_array_strided(self.Aflat, batch, self.sa)
+            self.pB = S.pointer_array_strided(self.Bflat, batch, self.sb)
+            self.pC = S.pointer_array_strided(self.Cflat, batch, self.sc)
         torch.cuda.synchronize()
+
+    def instance(self, which, i):
+        """Device tensor of instance i of operand 'A', 'B' or 'C'."""
+        if self.A is not None:
+            return {"A": self.A, "B": self.B, "C": self.C}[which][i]
+        return {"A": self.Aflat, "B": self.Bflat, "C": self.Cflat}[which][i]
+
+    def host(self, which, idx):
+        """Host copies (numpy) of the instances idx of one operand."""
+        import torch
+
+        if self.A is not None:
+            return [self.instance(which, int(i)).cpu().numpy() for i in idx]
+        flat = {"A": self.Aflat, "B": self.Bflat, "C": self.Cflat}[which]
+        rows = flat[torch.as_tensor(np.asarray(list(idx), dtype=np.int64), device=flat.device)].cpu().numpy()
+        return list(rows)
+
+    def save_c(self):
+        import torch
+
+        if self.A is not None:
+            return [c.clone() for c in self.C]
+        return self.Cflat.clone()
+
+    def restore_c(self, saved):
+        if self.A is not None:
+            for c, s in zip(self.C, saved):
+                c.copy_(s)
+        else:
+            self.Cflat.copy_(saved)
 
     def call(self, compute_type=0, stream=None):
         import ssblasex_amd as S
@@ -144,6 +216,12 @@ class Problem:
                                         self.ldc, self.batch, compute_type, 0, stream)
         if st != 0:
             raise RuntimeError(f"ssblasGemmBatchedExAsync returned {S.Status(st).name}")
+
+    def kernel(self, compute_type=0):
+        import ssblasex_amd as S
+
+        return S.kernel_config(self.ta, self.tb, self.m, self.n, self.k, self.ab, self.ab, self.cc, self.lda, self.ldb,
+                               self.ldc, self.batch, compute_type)
 
     def work(self):
         return 2.0 * self.m * self.n * self.k * self.batch
@@ -155,85 +233,248 @@ class Problem:
         return per * self.batch + 24 * self.batch
 
 
-def cpu_baseline_i8(prob: Problem, threads: int):
-    """The example's OpenMP int8 loop (restated in the oracle) on the host
-    cores: whole batch, repeated until >= 10 s or 5 runs."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+# ---------------------------------------------------------------------------
+# CPU leg: the reference example's baselines on the host cores, and the
+# oracle as the checker of the GPU results' errors
+# ---------------------------------------------------------------------------
+def host_info(threads):
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    return {"nproc": os.cpu_count(), "affinity_cpus": affinity, "cpu_model": model, "threads": threads,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_threads(args):
+    """Threads for the CPU baselines: every CPU this process may run on,
+    capped at the host share of one GPU (OMP_NUM_THREADS where it is set)."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    cap = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return min(n, cap) if cap > 0 else n
+
+
+def _oracle():
+    sys.path.insert(0, ROOT)
     import oracle
 
-    n_inst = min(prob.batch, len(prob.hostA))
-    A, B = prob.hostA[:n_inst], prob.hostB[:n_inst]
-    C = [c.copy() for c in prob.hostC[:n_inst]]
-    oracle.naive_baseline_i8(prob.ta, prob.tb, prob.m, prob.n, prob.k, prob.alpha, A, prob.lda, B, prob.ldb,
-                             prob.beta, C, prob.ldc, threads)  # warm-up
-    times = []
-    t_total = 0.0
-    while len(times) < 5 and t_total < 10.0:
+    return oracle
+
+
+def timed_runs(fn, budget_s=10.0, runs=5):
+    """One warm-up call, then calls until `runs` or `budget_s`; median seconds."""
+    fn()
+    times, total = [], 0.0
+    while len(times) < runs and total < budget_s:
         t0 = time.perf_counter()
-        oracle.naive_baseline_i8(prob.ta, prob.tb, prob.m, prob.n, prob.k, prob.alpha, A, prob.lda, B, prob.ldb,
-                                 prob.beta, C, prob.ldc, threads)
+        fn()
         dt = time.perf_counter() - t0
         times.append(dt)
-        t_total += dt
-    med = float(np.median(times))
-    ops = 2.0 * prob.m * prob.n * prob.k * n_inst
-    return {"value": ops / med / 1e12, "unit": "TOPS", "cores": threads, "kind": "port",
-            "sample": f"{n_inst} of {prob.batch} instances, the example's OpenMP int8 loop "
-                      f"(example/include/naive_gemm.hpp:3-155) restated in oracle/ssblas_oracle.c; "
-                      f"median of {len(times)} runs, {med:.3f} s each"}
+        total += dt
+    return float(np.median(times)), len(times)
 
 
-# The other benchmark configurations: (case id, name, problem, forms).
-SECONDARY_ROWS = [
-    ("c1", "fp32 NN m=n=k=64 batch 128 beta=0.5", dict(t="f32", ta=0, tb=0, m=64, n=64, k=64, batch=128, alpha=1.0,
-                                                       beta=0.5, seed=SEED + 1), [("native", 0, "f32")]),
-    ("c3", "fp64 NT m=n=k=32 batch 65536", dict(t="f64", ta=0, tb=1, m=32, n=32, k=32, batch=65536, alpha=1.0,
-                                                beta=0.0, seed=SEED + 3, dist="wide"),
-     [("native", 0, "f64"), ("ozaki_s6", 0x106, "f64"), ("ozaki_s7", 0x107, "f64"), ("ozaki_s8", 0x108, "f64"),
-      ("ozaki_s9", 0x109, "f64")]),
-    ("c3l", "fp64 NN m=n=k=256 batch 512", dict(t="f64", ta=0, tb=0, m=256, n=256, k=256, batch=512, alpha=1.0,
-                                                beta=0.0, seed=SEED + 5, dist="wide"),
-     [("native", 0, "f64"), ("ozaki_s7", 0x107, "f64"), ("ozaki_s8", 0x108, "f64"), ("ozaki_s9", 0x109, "f64")]),
-    ("c4", "fp32 TN m=n=k=256 batch 4096 alpha=1.5 beta=-0.5",
-     dict(t="f32", ta=1, tb=0, m=256, n=256, k=256, batch=4096, alpha=1.5, beta=-0.5, seed=SEED + 4,
-          dist="normal"), [("native", 0, "f32"), ("bf16x3", 4, "bf16"), ("bf16x6", 5, "bf16")]),
-    ("c4p", "fp32 TN m=n=k=256 batch 4096 alpha=1.5 beta=-0.5, lda=ldb=ldc=264",
-     dict(t="f32", ta=1, tb=0, m=256, n=256, k=256, batch=4096, alpha=1.5, beta=-0.5, seed=SEED + 4,
-          dist="normal", pad=8), [("native", 0, "f32"), ("bf16x3", 4, "bf16"), ("bf16x6", 5, "bf16")]),
-    ("c2b1", "int8 TT m=n=k=512 batch 64 beta=1",
-     dict(t="i8", ta=1, tb=1, m=512, n=512, k=512, batch=64, alpha=1, beta=1, seed=SEED + 2), [("native", 0, "i8")]),
+def cpu_baseline_i8(prob, threads):
+    """The example's OpenMP int8 loop (example/include/naive_gemm.hpp:3-155,
+    restated in the oracle) over the whole batch."""
+    oracle = _oracle()
+    idx = range(prob.batch)
+    A, B, C = prob.host("A", idx), prob.host("B", idx), prob.host("C", idx)
+
+    def run():
+        oracle.naive_baseline_i8(prob.ta, prob.tb, prob.m, prob.n, prob.k, prob.alpha, A, prob.lda, B, prob.ldb,
+                                 prob.beta, C, prob.ldc, threads)
+
+    med, n = timed_runs(run)
+    ops = 2.0 * prob.m * prob.n * prob.k * prob.batch
+    return {"value": round(ops / med / 1e12, 5), "unit": "TOPS", "cores": threads, "kind": "port",
+            "sample": f"all {prob.batch} instances, the example's OpenMP int8 loop (example/include/naive_gemm.hpp:3-155) "
+                      f"restated in oracle/ssblas_oracle.c; median of {n} runs, {med:.3f} s each"}
+
+
+def cpu_baseline_blas(prob, threads, max_instances):
+    """The example's fp baseline (example/include/naive_gemm.hpp:195-226): a
+    loop of cblas_sgemm / cblas_dgemm calls, one per instance, OpenBLAS
+    threads = `threads`, on the first `max_instances` instances."""
+    oracle = _oracle()
+    lib = oracle.find_openblas()
+    if lib is None:
+        return {"error": "scipy's OpenBLAS not found"}
+    nb = min(prob.batch, max_instances)
+    idx = range(nb)
+    A, B, C = prob.host("A", idx), prob.host("B", idx), prob.host("C", idx)
+
+    def run():
+        if oracle.blas_baseline(lib, prob.ab, prob.ta, prob.tb, prob.m, prob.n, prob.k, prob.alpha, A, prob.lda, B,
+                                prob.ldb, prob.beta, C, prob.ldc, threads) != 0:
+            raise RuntimeError("cblas symbols missing")
+
+    med, n = timed_runs(run)
+    flops = 2.0 * prob.m * prob.n * prob.k * nb
+    return {"value": round(flops / med / 1e12, 5), "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"{nb} of {prob.batch} instances, the example's per-instance cblas_{'s' if prob.t == 'f32' else 'd'}"
+                      f"gemm loop (example/include/naive_gemm.hpp:195-226) into scipy's OpenBLAS "
+                      f"({os.path.basename(lib)}); median of {n} runs, {med:.3f} s each"}
+
+
+class ErrorCheck:
+    """Normwise error ||C - C_exact||_F / (|alpha| ||A|| ||B|| + |beta| ||C0||),
+    worst over a seeded sample of instances, with C_exact from the
+    double-double oracle; the reference-order restatement (the oracle) on the
+    same sample gives the reference's own error for the ratio."""
+
+    def __init__(self, prob, nsample, threads):
+        oracle = _oracle()
+        self.prob, self.threads = prob, threads
+        rng = np.random.default_rng(SEED)
+        self.idx = np.sort(rng.choice(prob.batch, min(nsample, prob.batch), replace=False))
+        self.A, self.B, self.C0 = prob.host("A", self.idx), prob.host("B", self.idx), prob.host("C", self.idx)
+        p = prob
+        self.hi, self.lo = oracle.gemm_dd_batched(p.ta, p.tb, p.m, p.n, p.k, p.alpha, self.A, p.ab, p.lda, self.B,
+                                                  p.ldb, p.beta, self.C0, p.ldc, nthreads=threads)
+
+        def frob(bufs, rows, cols, ld):
+            return np.array([np.linalg.norm(oracle.colmajor(x, rows, cols, ld).astype(np.float64)) for x in bufs])
+
+        self.scale = abs(p.alpha) * frob(self.A, p.ra, p.ca, p.lda) * frob(self.B, p.rb, p.cb, p.ldb)
+        if p.beta != 0:
+            self.scale = self.scale + abs(p.beta) * frob(self.C0, p.m, p.n, p.ldc)
+        self.c0_dev = prob.save_c()  # every checked call starts from these C values
+        ref = [c.copy() for c in self.C0]
+        oracle.gemm_batched_ex(p.ta, p.tb, p.m, p.n, p.k, p.alpha, self.A, p.ab, p.lda, self.B, p.ab, p.ldb, p.beta,
+                               ref, p.cc, p.ldc, nthreads=threads)
+        self.ref_err = self.error(ref)
+
+    def error(self, C):
+        oracle = _oracle()
+        p = self.prob
+        worst = 0.0
+        for j, c in enumerate(C):
+            d = (oracle.colmajor(c, p.m, p.n, p.ldc).astype(np.float64) - self.hi[j]) - self.lo[j]
+            worst = max(worst, float(np.sqrt(np.sum(d * d)) / self.scale[j]))
+        return worst
+
+    def gpu_error(self, ct, stream):
+        """Error of one call from the initial C (timed steps with beta != 0
+        change C, so it is put back first)."""
+        import torch
+
+        p = self.prob
+        p.restore_c(self.c0_dev)
+        p.call(ct, stream)
+        torch.cuda.synchronize()
+        return self.error(p.host("C", self.idx))
+
+
+# ---------------------------------------------------------------------------
+# the configurations
+# ---------------------------------------------------------------------------
+C1 = dict(t="f32", ta=0, tb=0, m=64, n=64, k=64, batch=128, alpha=1.0, beta=0.5, seed=SEED + 1)
+C2B1 = dict(t="i8", ta=1, tb=1, m=512, n=512, k=512, batch=64, alpha=1, beta=1, seed=SEED + 2)
+C3 = dict(t="f64", ta=0, tb=1, m=32, n=32, k=32, batch=65536, alpha=1.0, beta=0.0, seed=SEED + 3, dist="wide",
+          separate=False)
+C3L = dict(t="f64", ta=0, tb=0, m=256, n=256, k=256, batch=512, alpha=1.0, beta=0.0, seed=SEED + 5, dist="wide",
+           separate=False)
+C4 = dict(t="f32", ta=1, tb=0, m=256, n=256, k=256, batch=4096, alpha=1.5, beta=-0.5, seed=SEED + 4, dist="normal",
+          separate=False)
+C4P = dict(C4, pad=8)
+FORMS_C4 = [("native", 0), ("bf16x3", 4), ("bf16x6", 5)]
+FORMS_C3 = [("native", 0), ("ozaki_s9", 0x109), ("ozaki_auto", 6)]
+# (case id, label, problem, forms, error sample size, cpu baseline instances, in the default line)
+CONFIGS = [
+    ("c4", "fp32 TN m=n=k=256 batch 4096 alpha=1.5 beta=-0.5", C4, FORMS_C4, 8, 512, True),
+    ("c3", "fp64 NT m=n=k=32 batch 65536 U[-1,1]*2^U{-8..8}", C3,
+     FORMS_C3 + [(f"ozaki_s{s}", 0x100 + s) for s in (6, 7, 8)], 1024, 65536, True),
+    ("c1", "fp32 NN m=n=k=64 batch 128 beta=0.5, separate allocations", C1, [("native", 0)], 128, 128, True),
+    ("c2b1", "int8 TT m=n=k=512 batch 64 beta=1", C2B1, [("native", 0)], 0, 0, True),
+    ("c4p", "fp32 TN m=n=k=256 batch 4096 alpha=1.5 beta=-0.5, lda=ldb=ldc=264", C4P, FORMS_C4, 8, 0, False),
+    ("c3l", "fp64 NN m=n=k=256 batch 512 U[-1,1]*2^U{-8..8}", C3L,
+     [("native", 0), ("ozaki_s8", 0x108), ("ozaki_s9", 0x109), ("ozaki_auto", 6)], 8, 0, False),
 ]
+DEFAULT_FORMS = {"c3": {"native", "ozaki_s9", "ozaki_auto"}}
 
 
-def secondary(args, stream, only=None):
-    """Other benchmark configurations: native vs split forms.  `only` =
-    (case id, form name) runs just that one (profiling runs)."""
-    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+def run_config(cid, args, stream, forms=None, cpu=True, threads=1):
+    """Time each form of one configuration; with `cpu`, also its error on
+    the seeded sample and the CPU baseline."""
     import torch
 
+    _, label, kw, all_forms, nsample, cpu_inst, _ = next(c for c in CONFIGS if c[0] == cid)
+    prob = Problem(**kw)
+    check = ErrorCheck(prob, nsample, threads) if (cpu and nsample and kw["t"] != "i8") else None
+    steps = max(20, args.steps // 10)
+    res = {"label": label, "forms": {}}
+    if check is not None:
+        res["reference_order_error"] = check.ref_err
+        res["error_sample"] = f"{len(check.idx)} of {prob.batch} instances (seeded)"
+    for fname, ct in all_forms:
+        if forms is not None and fname not in forms:
+            continue
+        err = check.gpu_error(ct, stream) if check is not None else None
+        fn = (lambda: prob.call(ct, stream))
+        try:
+            warm_for(fn, 0.2, stream)
+            ms = time_steps(fn, steps, 3, stream)
+        except RuntimeError as e:
+            res["forms"][fname] = {"error": str(e)}
+            continue
+        sec = ms * 1e-3
+        rate = prob.work() / sec / 1e12
+        gbs = prob.bytes() / sec / 1e9
+        peak = PEAK["int8" if kw["t"] == "i8" else kw["t"]]
+        ent = {"kernel": prob.kernel(ct), "ms_per_step": round(ms, 5),
+               ("TOPS" if kw["t"] == "i8" else "TFLOPs"): round(rate, 3),
+               "frac_of_native_mfma_peak": round(rate / peak, 4), "hbm_GBs": round(gbs, 1),
+               "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+        if err is not None:
+            ent["normwise_error"] = err
+            ent["ratio_to_reference_order"] = round(err / check.ref_err, 4) if check.ref_err > 0 else None
+        res["forms"][fname] = ent
+    if cpu and cpu_inst:
+        res["cpu_baseline"] = cpu_baseline_blas(prob, threads, cpu_inst)
+    del prob, check
+    torch.cuda.empty_cache()
+    return res
+
+
+def split_vs_native(args, stream, threads, ids, cpu):
+    """The metric's second half: split-precision forms against native on the
+    same batch, with their errors."""
     out = {}
-    for cid, name, kw, forms in SECONDARY_ROWS:
-        if only is not None:
-            if cid != only[0]:
-                continue
-            forms = [f for f in forms if f[0] == only[1]]
-            if not forms:
-                raise SystemExit(f"case {cid} has no form {only[1]}")
-        p = Problem(**kw)
-        res = {}
-        for fname, ct, peak_key in forms:
-            try:
-                ms = time_steps(lambda: p.call(ct, stream), max(5, args.steps // 10), 3, stream)
-            except RuntimeError as e:
-                res[fname] = {"error": str(e)}
-                continue
-            v = p.work() / (ms * 1e-3) / 1e12
-            res[fname] = {"TFLOPs" if kw["t"] != "i8" else "TOPS": round(v, 3), "ms_per_step": round(ms, 4),
-                          "frac_of_native_peak": round(v / PEAK[kw["t"] if kw["t"] != "i8" else "int8"], 4),
-                          "hbm_GBs": round(p.bytes() / (ms * 1e-3) / 1e9, 1)}
-        out[name] = res
-        del p
-        torch.cuda.empty_cache()
+    for cid in ids:
+        forms = DEFAULT_FORMS.get(cid) if not args.secondary else None
+        out[cid] = run_config(cid, args, stream, forms=forms, cpu=cpu, threads=threads)
+    # the comparisons the metric names, read off the entries above
+    summ = {}
+    if "c4" in out and "native" in out["c4"]["forms"]:
+        f = out["c4"]["forms"]
+        for s in ("bf16x3", "bf16x6"):
+            if s in f and "ms_per_step" in f[s]:
+                summ[f"c4 {s} speedup over native fp32"] = round(f["native"]["ms_per_step"] / f[s]["ms_per_step"], 3)
+                if "ratio_to_reference_order" in f[s]:
+                    summ[f"c4 {s} error / native error"] = round(f[s]["normwise_error"] / f["native"]["normwise_error"], 3)
+    if "c3" in out and "native" in out["c3"]["forms"]:
+        f = out["c3"]["forms"]
+        for s in ("ozaki_s9", "ozaki_auto"):
+            if s in f and "ms_per_step" in f[s]:
+                summ[f"c3 {s} speedup over native fp64"] = round(f["native"]["ms_per_step"] / f[s]["ms_per_step"], 3)
+                if "normwise_error" in f[s]:
+                    summ[f"c3 {s} error / native error"] = round(f[s]["normwise_error"] / f["native"]["normwise_error"], 3)
+    out["summary"] = summ
     return out
 
 
@@ -335,13 +576,14 @@ def grouped_c5(args, stream, only=None):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (batches shard by "
-                                   "instance with no exchange; see DESIGN.md)"}))
+        print(json.dumps({"error": "this build runs on one GPU: --gpus must be 1 (batches shard by instance with "
+                                   "no exchange step; a multi-GPU run is not part of this build)"}))
         sys.exit(2)
     import torch
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     stream = torch.cuda.current_stream()
+    threads = cpu_threads(args)
     if args.copy_probe:
         n = args.copy_probe << 20
         src = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
@@ -355,19 +597,32 @@ def main():
         if cid == "c5":
             print(json.dumps({"case": args.case, "result": grouped_c5(args, stream, only=form or "all")}), flush=True)
         else:
-            print(json.dumps({"case": args.case, "result": secondary(args, stream, only=(cid, form))}), flush=True)
+            forms = {form} if form else None
+            print(json.dumps({"case": args.case, "result": run_config(cid, args, stream, forms=forms, cpu=False)}),
+                  flush=True)
         return
+    # the headline batch is drawn first, so that its warm-up follows the
+    # other configurations' device work without an idle gap
     prob = Problem("i8", 1, 1, 512, 512, 512, 64, alpha=1, beta=args.beta, seed=SEED + 2)
+    line_extra = {}
+    if not args.no_split:
+        ids = [c[0] for c in CONFIGS if c[6] or args.secondary]
+        line_extra["split_vs_native"] = split_vs_native(args, stream, threads, ids, cpu=not args.no_cpu_baseline)
+    if args.grouped or args.secondary:
+        line_extra["grouped_c5"] = grouped_c5(args, stream)
+
     ms = time_steps(lambda: prob.call(0, stream), args.steps, args.warmup, stream)
     sec = ms * 1e-3
     tops = prob.work() / sec / 1e12
     q = prob.bytes()
     achieved_gbs = q / sec / 1e9
-    traffic = None
-    prof = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "dma_stream_gemm_i8_TT_512x64_traffic.json")
+    traffic, traffic_src = None, None
+    prof = os.path.join(ROOT, "profiles", "dma_stream_gemm_i8_TT_512x64_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
-            traffic = json.load(f).get("beta%d" % args.beta)
+            tj = json.load(f)
+        traffic = tj.get("beta%d" % args.beta)
+        traffic_src = "profiles/dma_stream_gemm_i8_TT_512x64_traffic.json: " + tj.get("_how", "")
     line = {
         "metric": METRIC,
         "value": round(tops, 3),
@@ -384,18 +639,16 @@ def main():
         "config": {"workload": "ssblasGemmBatchedEx int8->int32 TT m=n=k=512 batchCount=64 lda=ldb=ldc=512 "
                                f"alpha=1 beta={args.beta}, separate allocations, device pointer arrays",
                    "batchCount": 64, "m": 512, "n": 512, "k": 512, "beta": args.beta},
+        "kernel": prob.kernel(0),
         "mfma_frac": round(tops / PEAK["int8"], 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "algorithmic_bytes_per_launch": q},
+                     "traffic_source": traffic_src, "algorithmic_bytes_per_launch": q},
     }
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline_i8(prob, threads)
-    if args.secondary:
-        line["secondary"] = secondary(args, stream)
-    if args.grouped or args.secondary:
-        line["grouped_c5"] = grouped_c5(args, stream)
+        line["cpu_baseline"]["host"] = host_info(threads)
+    line.update(line_extra)
     print(json.dumps(line), flush=True)
 
 

@@ -209,6 +209,31 @@ ssblasStatus_t ssblasGemmGroupedBatchedEx(
     ssblasComputeType_t      computeType,
     ssblasGemmAlgo_t         algo);
 
+/*
+ * Which kernel a call with these arguments would run (no counterpart in the
+ * reference; for tests and tuning): writes the tile-config name (e.g.
+ * "dma256k2", "tile256x128", "direct16") or the Ozaki kernel family
+ * ("ozaki_small", "ozaki_gemm") into name[0 .. name_len-1].  Same status
+ * table as ssblasGemmBatchedEx for the arguments it takes; touches no
+ * device memory.
+ */
+ssblasStatus_t ssblasGetKernelConfig(
+    ssblasOperation_t    transa,
+    ssblasOperation_t    transb,
+    SSINT                m,
+    SSINT                n,
+    SSINT                k,
+    ssblasDataType_t     Atype,
+    ssblasDataType_t     Btype,
+    ssblasDataType_t     Ctype,
+    const SSINT          lda,
+    const SSINT          ldb,
+    const SSINT          ldc,
+    const SSINT          batchCount,
+    ssblasComputeType_t  computeType,
+    char*                name,
+    SSINT                name_len);
+
 /* Library version as MAJOR*10000 + MINOR*100 + PATCH. */
 int ssblasGetVersion(void);
 

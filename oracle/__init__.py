@@ -10,7 +10,9 @@ reference file:line each function restates) plus numpy helpers:
 * gemm_batched_ex   restatement of the reference's ssblasGemmBatchedEx
                     (status table + exact fp operation order), host arrays
 * gemm_dd           double-double FP64 oracle, returns (hi, lo)
+* gemm_dd_batched   the same over a batch, parallel over instances
 * naive_baseline_i8 the example's OpenMP int8 baseline loop
+* blas_baseline     the example's per-instance cblas_?gemm loop (scipy's OpenBLAS)
 * colmajor / normwise_error  helpers for column-major buffers
 
 Parity status of each function is recorded in tests/test_oracle.py, which
@@ -53,6 +55,12 @@ def load() -> ctypes.CDLL:
         lib.oracle_gemm_dd.argtypes = [I, I, L, L, L, ctypes.c_double, V, I, L, V, L, ctypes.c_double, V, L, V, V]
         lib.oracle_naive_baseline_i8.restype = None
         lib.oracle_naive_baseline_i8.argtypes = [I, I, L, L, L, ctypes.c_int32, V, L, V, L, ctypes.c_int32, V, L, L, I]
+        lib.oracle_gemm_dd_batched.restype = None
+        lib.oracle_gemm_dd_batched.argtypes = [I, I, L, L, L, ctypes.c_double, V, I, L, V, L, ctypes.c_double, V, L,
+                                               V, V, L, I]
+        lib.oracle_blas_baseline.restype = I
+        lib.oracle_blas_baseline.argtypes = [ctypes.c_char_p, I, I, I, L, L, L, ctypes.c_double, V, L, V, L,
+                                             ctypes.c_double, V, L, L, I]
         lib.oracle_max_threads.restype = I
         _lib = lib
     return _lib
@@ -95,6 +103,42 @@ def gemm_dd(transa, transb, m, n, k, alpha, A, Atype, lda, B, ldb, beta, C0, ldc
     load().oracle_gemm_dd(transa, transb, m, n, k, float(alpha), _ptr(A), Atype, lda, _ptr(B), ldb,
                           float(beta), _ptr(c0), ldc, _ptr(hi), _ptr(lo))
     return hi.reshape(n, m).T, lo.reshape(n, m).T
+
+
+def gemm_dd_batched(transa, transb, m, n, k, alpha, A, Atype, lda, B, ldb, beta, C0, ldc, nthreads=1):
+    """oracle_gemm_dd over lists of host buffers, parallel over instances ->
+    (hi, lo), each (batch, m, n) float64."""
+    batch = len(A)
+    hi = np.empty(batch * m * n, dtype=np.float64)
+    lo = np.empty(batch * m * n, dtype=np.float64)
+    pa, pb = _ptr_array(A), _ptr_array(B)
+    pc = _ptr_array(C0 if beta != 0 else A)  # C0 unread when beta == 0
+    load().oracle_gemm_dd_batched(transa, transb, m, n, k, float(alpha), ctypes.addressof(pa), Atype, lda,
+                                  ctypes.addressof(pb), ldb, float(beta), ctypes.addressof(pc), ldc, _ptr(hi),
+                                  _ptr(lo), batch, nthreads)
+    return hi.reshape(batch, n, m).transpose(0, 2, 1), lo.reshape(batch, n, m).transpose(0, 2, 1)
+
+
+def find_openblas():
+    """Path of the OpenBLAS shipped with scipy (scipy.libs/libscipy_openblas*.so), or None."""
+    import glob
+    import importlib.util
+
+    spec = importlib.util.find_spec("scipy")
+    if spec is None or spec.origin is None:
+        return None
+    site = Path(spec.origin).resolve().parent.parent
+    hits = sorted(glob.glob(str(site / "scipy.libs" / "libscipy_openblas*.so")))
+    return hits[0] if hits else None
+
+
+def blas_baseline(libpath, Atype, transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nthreads) -> int:
+    """The example's per-instance cblas_?gemm loop (C updated in place); -1
+    when the library or its symbols are missing."""
+    pa, pb, pc = _ptr_array(A), _ptr_array(B), _ptr_array(C)
+    return load().oracle_blas_baseline(str(libpath).encode(), Atype, transa, transb, m, n, k, float(alpha),
+                                       ctypes.addressof(pa), lda, ctypes.addressof(pb), ldb, float(beta),
+                                       ctypes.addressof(pc), ldc, len(C), nthreads)
 
 
 def naive_baseline_i8(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nthreads):

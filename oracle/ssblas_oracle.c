@@ -34,12 +34,16 @@
  *    alpha*sum_k a*b + beta*c evaluated in double-double (Dot2 of Ogita,
  *    Rump and Oishi, SIAM J. Sci. Comput. 26(6), 2005, built on Knuth's
  *    TwoSum and an fma-based TwoProduct), returned as an unevaluated
- *    hi + lo pair.  Independent of the reference.
+ *    hi + lo pair.  Independent of the reference.  oracle_gemm_dd_batched:
+ *    the same per element, over a batch, parallel over instances.
  *
  * 3. oracle_naive_baseline_i8 -- the example's baseline loop for int8
  *    (example/include/naive_gemm.hpp:3-155): OpenMP collapse(3) over
  *    (instance, j, i), int accumulator, C = alpha*tmp + beta*C when
  *    beta != 0 else alpha*tmp.  Used as the timed CPU baseline.
+ *    oracle_blas_baseline -- the example's fp baseline
+ *    (example/include/naive_gemm.hpp:195-226), a per-instance loop of
+ *    cblas_sgemm / cblas_dgemm calls into an OpenBLAS opened at run time.
  *
  * Build: see oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off, so that
  * the compiler neither fuses nor splits any operation written here).
@@ -49,6 +53,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <omp.h>
+#include <dlfcn.h>
 
 #include "../include/ssblasBatch_type.h"
 
@@ -215,6 +220,42 @@ static inline void two_prod(double a, double b, double* p, double* e)
     *p = x;
 }
 
+/* One element of the double-double oracle: (*hi, *lo) ~= alpha * sum_k
+ * op(A)(i,kk) * op(B)(kk,j) + beta * C0(i,j). */
+static void dd_element(ssblasOperation_t ta, ssblasOperation_t tb, idx_t i, idx_t j, idx_t k,
+                       double alpha, const void* A, ssblasDataType_t type, idx_t lda,
+                       const void* B, idx_t ldb, double beta, const void* C0, idx_t ldc,
+                       double* hi, double* lo)
+{
+    double s = 0.0, t = 0.0; /* running (s + t) */
+    for (idx_t kk = 0; kk < k; ++kk) {
+        idx_t ia = op_index(ta, i, kk, lda), ib = op_index(tb, kk, j, ldb);
+        double a, b;
+        if (type == SSBLAS_R_32F) { a = ((const float*)A)[ia]; b = ((const float*)B)[ib]; }
+        else if (type == SSBLAS_R_64F) { a = ((const double*)A)[ia]; b = ((const double*)B)[ib]; }
+        else { a = ((const int8_t*)A)[ia]; b = ((const int8_t*)B)[ib]; }
+        double h, r, q;
+        two_prod(a, b, &h, &r);
+        two_sum(s, h, &s, &q);
+        t += q + r;
+    }
+    /* (s, t) * alpha */
+    double ph, pe;
+    two_prod(s, alpha, &ph, &pe);
+    pe = fma(t, alpha, pe);
+    if (beta != 0.0) {
+        double c;
+        if (type == SSBLAS_R_32F) c = ((const float*)C0)[i + j * ldc];
+        else if (type == SSBLAS_R_64F) c = ((const double*)C0)[i + j * ldc];
+        else c = ((const int32_t*)C0)[i + j * ldc];
+        double bh, be, q;
+        two_prod(beta, c, &bh, &be);
+        two_sum(ph, bh, &ph, &q);
+        pe += q + be;
+    }
+    two_sum(ph, pe, hi, lo);
+}
+
 /*
  * hi[i] + lo[i] ~= alpha * sum_k op(A)(r,kk) * op(B)(kk,c) + beta * C0(r,c)
  * for one instance, with the element type given by `type` (R_32F / R_64F /
@@ -227,39 +268,32 @@ void oracle_gemm_dd(ssblasOperation_t ta, ssblasOperation_t tb, idx_t m, idx_t n
                     double* hi, double* lo)
 {
 #pragma omp parallel for collapse(2) schedule(static)
-    for (idx_t j = 0; j < n; ++j) {
-        for (idx_t i = 0; i < m; ++i) {
-            double s = 0.0, t = 0.0; /* running (s + t) */
-            for (idx_t kk = 0; kk < k; ++kk) {
-                idx_t ia = op_index(ta, i, kk, lda), ib = op_index(tb, kk, j, ldb);
-                double a, b;
-                if (type == SSBLAS_R_32F) { a = ((const float*)A)[ia]; b = ((const float*)B)[ib]; }
-                else if (type == SSBLAS_R_64F) { a = ((const double*)A)[ia]; b = ((const double*)B)[ib]; }
-                else { a = ((const int8_t*)A)[ia]; b = ((const int8_t*)B)[ib]; }
-                double h, r, q;
-                two_prod(a, b, &h, &r);
-                two_sum(s, h, &s, &q);
-                t += q + r;
-            }
-            /* (s, t) * alpha */
-            double ph, pe;
-            two_prod(s, alpha, &ph, &pe);
-            pe = fma(t, alpha, pe);
-            if (beta != 0.0) {
-                double c;
-                if (type == SSBLAS_R_32F) c = ((const float*)C0)[i + j * ldc];
-                else if (type == SSBLAS_R_64F) c = ((const double*)C0)[i + j * ldc];
-                else c = ((const int32_t*)C0)[i + j * ldc];
-                double bh, be, q;
-                two_prod(beta, c, &bh, &be);
-                two_sum(ph, bh, &ph, &q);
-                pe += q + be;
-            }
-            double h2, e2;
-            two_sum(ph, pe, &h2, &e2);
-            hi[i + j * m] = h2;
-            lo[i + j * m] = e2;
-        }
+    for (idx_t j = 0; j < n; ++j)
+        for (idx_t i = 0; i < m; ++i)
+            dd_element(ta, tb, i, j, k, alpha, A, type, lda, B, ldb, beta, C0, ldc, &hi[i + j * m],
+                       &lo[i + j * m]);
+}
+
+/*
+ * oracle_gemm_dd over a batch given as host pointer arrays, parallel over
+ * instances (for checks of whole benchmark batches of small instances):
+ * instance b's result goes to hi/lo + b*m*n, dense column-major m x n.
+ * Every element is computed by the same dd_element as oracle_gemm_dd.
+ */
+void oracle_gemm_dd_batched(ssblasOperation_t ta, ssblasOperation_t tb, idx_t m, idx_t n, idx_t k,
+                            double alpha, const void* const A[], ssblasDataType_t type, idx_t lda,
+                            const void* const B[], idx_t ldb, double beta, const void* const C0[],
+                            idx_t ldc, double* hi, double* lo, idx_t batch, int nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads)
+    for (idx_t bt = 0; bt < batch; ++bt) {
+        double* h = hi + bt * m * n;
+        double* l = lo + bt * m * n;
+        for (idx_t j = 0; j < n; ++j)
+            for (idx_t i = 0; i < m; ++i)
+                dd_element(ta, tb, i, j, k, alpha, A[bt], type, lda, B[bt], ldb, beta,
+                           beta != 0.0 ? C0[bt] : NULL, ldc, &h[i + j * m], &l[i + j * m]);
     }
 }
 
@@ -295,6 +329,52 @@ void oracle_naive_baseline_i8(ssblasOperation_t ta, ssblasOperation_t tb, idx_t 
             }
         }
     }
+}
+
+/*
+ * The example's fp baseline (example/include/naive_gemm.hpp:195-226): a
+ * serial loop over the instances, one column-major cblas_sgemm /
+ * cblas_dgemm call each (same trans, sizes, leading dimensions, alpha and
+ * beta), the BLAS library threading each call.  The image has no cblas.h or
+ * system BLAS, so the OpenBLAS that ships with scipy is opened by path at
+ * run time: its CBLAS entry points carry the "scipy_" prefix, LP64 ints.
+ * Returns 0, or -1 when the library or a symbol is missing.
+ */
+typedef void (*sgemm_fn)(int, int, int, int, int, int, float, const float*, int, const float*, int,
+                         float, float*, int);
+typedef void (*dgemm_fn)(int, int, int, int, int, int, double, const double*, int, const double*, int,
+                         double, double*, int);
+typedef void (*set_threads_fn)(int);
+
+int oracle_blas_baseline(const char* libpath, ssblasDataType_t type, ssblasOperation_t ta,
+                         ssblasOperation_t tb, idx_t m, idx_t n, idx_t k, double alpha,
+                         const void* const A[], idx_t lda, const void* const B[], idx_t ldb,
+                         double beta, void* const C[], idx_t ldc, idx_t batch, int nthreads)
+{
+    static void* h = NULL;
+    if (h == NULL) h = dlopen(libpath, RTLD_NOW | RTLD_LOCAL);
+    if (h == NULL) return -1;
+    set_threads_fn set_threads = (set_threads_fn)dlsym(h, "scipy_openblas_set_num_threads");
+    if (set_threads != NULL && nthreads > 0) set_threads(nthreads);
+    const int layout = 102; /* CblasColMajor */
+    const int opa = ta == SSBLAS_OP_T ? 112 : 111, opb = tb == SSBLAS_OP_T ? 112 : 111;
+    if (type == SSBLAS_R_32F) {
+        sgemm_fn f = (sgemm_fn)dlsym(h, "scipy_cblas_sgemm");
+        if (f == NULL) return -1;
+        for (idx_t bt = 0; bt < batch; ++bt)
+            f(layout, opa, opb, (int)m, (int)n, (int)k, (float)alpha, (const float*)A[bt], (int)lda,
+              (const float*)B[bt], (int)ldb, (float)beta, (float*)C[bt], (int)ldc);
+        return 0;
+    }
+    if (type == SSBLAS_R_64F) {
+        dgemm_fn f = (dgemm_fn)dlsym(h, "scipy_cblas_dgemm");
+        if (f == NULL) return -1;
+        for (idx_t bt = 0; bt < batch; ++bt)
+            f(layout, opa, opb, (int)m, (int)n, (int)k, alpha, (const double*)A[bt], (int)lda,
+              (const double*)B[bt], (int)ldb, beta, (double*)C[bt], (int)ldc);
+        return 0;
+    }
+    return -1;
 }
 
 int oracle_max_threads(void) { return omp_get_max_threads(); }

@@ -160,6 +160,17 @@ def ssblasGemmBatchedExAsync(transa, transb, m, n, k, alpha, A, Atype, lda, B, B
     return Status(_lib.load().ssblasGemmBatchedExAsync(*args, s))
 
 
+def kernel_config(transa, transb, m, n, k, Atype, Btype, Ctype, lda, ldb, ldc, batchCount,
+                  computeType=0) -> str:
+    """Name of the kernel (tile config or Ozaki family) the call would run."""
+    buf = ctypes.create_string_buffer(64)
+    st = _lib.load().ssblasGetKernelConfig(int(transa), int(transb), m, n, k, int(Atype), int(Btype), int(Ctype),
+                                           lda, ldb, ldc, batchCount, int(computeType), buf, 64)
+    if st != 0:
+        raise ValueError(f"ssblasGetKernelConfig returned {Status(st).name}")
+    return buf.value.decode()
+
+
 def version() -> int:
     return int(_lib.load().ssblasGetVersion())
 
@@ -174,10 +185,21 @@ def pointer_array(tensors: Sequence, device=None):
     return torch.tensor(ptrs, dtype=torch.int64, device=dev)
 
 
+def pointer_array_strided(base, count: int, stride_elems: int, offset_elems: int = 0):
+    """Device int64 tensor of `count` pointers into one allocation: element
+    offset_elems + i * stride_elems of the tensor `base` for instance i (a
+    batch held in one buffer, instance i at a fixed stride)."""
+    import torch
+
+    step = base.element_size()
+    start = int(base.data_ptr()) + offset_elems * step
+    return torch.arange(count, dtype=torch.int64, device=base.device) * (stride_elems * step) + start
+
+
 from .grouped import GroupedPlan, ssblasGemmGroupedBatchedEx  # noqa: E402,F401
 
 __all__ = [
     "Status", "Operation", "DataType", "ComputeType", "GemmAlgo", "scale_ctype",
     "ssblasGemmBatchedEx", "ssblasGemmBatchedEx_64", "ssblasGemmBatchedExAsync",
-    "ssblasGemmGroupedBatchedEx", "GroupedPlan", "pointer_array", "version",
+    "ssblasGemmGroupedBatchedEx", "GroupedPlan", "pointer_array", "pointer_array_strided", "kernel_config", "version",
 ]

@@ -45,6 +45,8 @@ SIGNATURES = {
     "ssblasGroupedPlanGraphLaunch": (_c_int, [_vp, _vp]),
     "ssblasGroupedPlanDestroy": (_c_int, [_vp]),
     "ssblasGemmGroupedBatchedEx": (_c_int, _GROUPED_ARGS),
+    "ssblasGetKernelConfig": (_c_int, [_c_int, _c_int, _c_long, _c_long, _c_long, _c_int, _c_int, _c_int, _c_long,
+                                       _c_long, _c_long, _c_long, _c_int, ctypes.c_char_p, _c_long]),
     "ssblasGetVersion": (_c_int, []),
 }
 

@@ -164,6 +164,23 @@ bool device_accessible(const void* p)
            attr.type == hipMemoryTypeHost || attr.type == hipMemoryTypeUnified;
 }
 
+size_t workspace_limit()
+{
+    // SSBLAS_WORKSPACE_LIMIT=<bytes>[K|M|G]: largest device workspace the
+    // library may allocate for one call or plan; beyond it the call returns
+    // SSBLAS_STATUS_ALLOC_FAILED before any device work.
+    const char* e = getenv("SSBLAS_WORKSPACE_LIMIT");
+    if (e == nullptr || *e == 0) return SIZE_MAX;
+    char* end = nullptr;
+    unsigned long long v = strtoull(e, &end, 10);
+    if (end != nullptr) {
+        if (*end == 'K' || *end == 'k') v <<= 10;
+        else if (*end == 'M' || *end == 'm') v <<= 20;
+        else if (*end == 'G' || *end == 'g') v <<= 30;
+    }
+    return (size_t)v;
+}
+
 ssblasStatus_t hip_status(hipError_t e)
 {
     if (e == hipSuccess) return SSBLAS_STATUS_SUCCESS;
@@ -211,8 +228,6 @@ ssblasStatus_t gemm_batched(ssblasOperation_t transa, ssblasOperation_t transb, 
     if (form < 0) return SSBLAS_STATUS_INVALID_VALUE;
     if (alpha == nullptr || beta == nullptr) return SSBLAS_STATUS_INVALID_VALUE;
     if (k == 0) return SSBLAS_STATUS_SUCCESS;  // no K block: C unchanged (reference ..._BMKN.hpp:200)
-    if (!device_accessible(A) || !device_accessible(B) || !device_accessible(C))
-        return SSBLAS_STATUS_INVALID_VALUE;
 
     GemmProblem g;
     g.ta = transa == SSBLAS_OP_T;
@@ -228,6 +243,11 @@ ssblasStatus_t gemm_batched(ssblasOperation_t transa, ssblasOperation_t transb, 
     g.B = B;
     g.C = C;
     fill_scalars(tt, alpha, beta, g);
+    // Library-owned workspace (the Ozaki form only) beyond the caller's cap.
+    if (form == FORM_OZAKI && ozaki_workspace_bytes(g, slices) > workspace_limit())
+        return SSBLAS_STATUS_ALLOC_FAILED;
+    if (!device_accessible(A) || !device_accessible(B) || !device_accessible(C))
+        return SSBLAS_STATUS_INVALID_VALUE;
 
     hipError_t e;
     if (form == FORM_OZAKI) {
@@ -287,6 +307,44 @@ ssblasStatus_t ssblasGemmBatchedExAsync(ssblasOperation_t transa, ssblasOperatio
                              batchCount, computeType, algo, reinterpret_cast<hipStream_t>(stream), false);
 }
 
-int ssblasGetVersion(void) { return 100; }
+ssblasStatus_t ssblasGetKernelConfig(ssblasOperation_t transa, ssblasOperation_t transb, SSINT m, SSINT n, SSINT k,
+                                     ssblasDataType_t Atype, ssblasDataType_t Btype, ssblasDataType_t Ctype,
+                                     const SSINT lda, const SSINT ldb, const SSINT ldc, const SSINT batchCount,
+                                     ssblasComputeType_t computeType, char* name, SSINT name_len)
+{
+    if (name == nullptr || name_len < 1) return SSBLAS_STATUS_INVALID_VALUE;
+    name[0] = 0;
+    ssb::TypeTriple tt = ssb::classify_types(Atype, Btype, Ctype);
+    if (tt == ssb::TT_NONE) return SSBLAS_STATUS_INVALID_VALUE;
+    ssblasStatus_t st = ssb::check_shape(transa, transb, m, n, k, lda, ldb, ldc, batchCount);
+    if (st != SSBLAS_STATUS_SUCCESS) return st;
+    int slices = 0;
+    const int form = ssb::select_form(tt, computeType, &slices);
+    if (form < 0) return SSBLAS_STATUS_INVALID_VALUE;
+    ssb::GemmProblem g{};
+    g.ta = transa == SSBLAS_OP_T;
+    g.tb = transb == SSBLAS_OP_T;
+    g.m = (int)m;
+    g.n = (int)n;
+    g.k = (int)k;
+    g.lda = lda;
+    g.ldb = ldb;
+    g.ldc = ldc;
+    g.batch = (int)batchCount;
+    const char* what;
+    char buf[64];
+    if (form == ssb::FORM_OZAKI) {
+        what = ssb::ozaki_kernel_name(g);
+    } else {
+        g.arith = form == ssb::FORM_I8 ? ssb::K_I8 : form == ssb::FORM_F32 ? ssb::K_F32
+                : form == ssb::FORM_F64 ? ssb::K_F64 : form == ssb::FORM_BF16X3 ? ssb::K_BF16X3 : ssb::K_BF16X6;
+        what = ssb::config_name(ssb::pick_config(g));
+    }
+    snprintf(buf, sizeof(buf), "%s", what);
+    snprintf(name, (size_t)name_len, "%s", buf);
+    return SSBLAS_STATUS_SUCCESS;
+}
+
+int ssblasGetVersion(void) { return 200; }
 
 }  // extern "C"

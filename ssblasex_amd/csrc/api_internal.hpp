@@ -16,6 +16,9 @@ ssblasStatus_t check_shape(ssblasOperation_t transa, ssblasOperation_t transb, l
                            long lda, long ldb, long ldc, long batchCount);
 bool device_accessible(const void* p);
 ssblasStatus_t hip_status(hipError_t e);
+// Cap on library-owned device workspace per call or plan
+// (SSBLAS_WORKSPACE_LIMIT, default unlimited).
+size_t workspace_limit();
 void fill_scalars(TypeTriple tt, const void* alpha, const void* beta, GemmProblem& g);
 
 // Tile config for a problem: the autotune table when it has an entry for
@@ -30,6 +33,22 @@ hipError_t launch_ozaki(const GemmProblem& g, int slices, hipStream_t stream, ss
 // Device side of the Ozaki form (ozaki.hip); ws_get returns a device buffer
 // of at least the requested size owned by the caller's stream.
 hipError_t ozaki_run(const GemmProblem& g, int s, void* (*ws_get)(size_t, hipError_t*), hipStream_t st);
+// Workspace of the general Ozaki path: the int8 slices of op(A) and op(B)
+// (each kept in its operand's storage orientation, leading dimension rounded
+// up to 16 bytes), then the int32 exponent vectors.
+struct OzakiLayout {
+    long ld_sa, cols_sa, ld_sb, cols_sb;  // slice matrices: storage ld and column count
+    size_t off_b, off_e, total;           // byte offsets of B's slices and the exponents; total bytes
+};
+OzakiLayout ozaki_layout(const GemmProblem& g, int s);
+// Device workspace an Ozaki call of this shape needs with s slices (0 = the
+// automatic choice), in bytes (0 for the fused form).
+size_t ozaki_workspace_bytes(const GemmProblem& g, int s);
+// The fused one-wave Ozaki form applies (m, n, k <= 32).
+bool ozaki_fused_shape(const GemmProblem& g);
+// Kernel family an Ozaki call of this shape runs ("ozaki_small" for the
+// fused one-wave form, "ozaki_gemm" otherwise).
+const char* ozaki_kernel_name(const GemmProblem& g);
 // Slice count used when the compute type does not name one.
 constexpr int OZAKI_DEFAULT_SLICES = 9;
 

@@ -6,6 +6,7 @@
 
 #include "api_internal.hpp"
 #include "launch.hpp"
+#include "occ_cache.hpp"
 
 namespace ssb {
 
@@ -50,6 +51,12 @@ void config_tile(int cfg, int* tm, int* tn)
     if (cfg < 0 || cfg >= CFG_COUNT) cfg = CFG_TILE_64;
     *tm = dims[cfg][0];
     *tn = dims[cfg][1];
+}
+
+OccupancyCache& occupancy_cache()
+{
+    static OccupancyCache cache;
+    return cache;
 }
 
 int device_cu_count()

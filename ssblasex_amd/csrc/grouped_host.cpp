@@ -233,8 +233,6 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
         }
     }
     if (A == nullptr || B == nullptr || C == nullptr) return SSBLAS_STATUS_INVALID_VALUE;
-    if (!device_accessible(A) || !device_accessible(B) || !device_accessible(C))
-        return SSBLAS_STATUS_INVALID_VALUE;
 
     // Work lists per family (workgroup items, wave items), longest first.
     const int other_family = fp32_form == K_BF16X3 ? GF_I8_BF16X3 : fp32_form == K_BF16X6 ? GF_I8_BF16X6 : GF_I8_F32;
@@ -267,6 +265,10 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
     const size_t off_insts = align(off_groups + groups.size() * sizeof(GGroup));
     const size_t off_items = align(off_insts + insts.size() * sizeof(GInst));
     const size_t bytes = off_items + (size_t)std::max(nitems, 1L) * sizeof(GItem);
+    // the plan's device image is library-owned workspace
+    if (bytes > workspace_limit()) return SSBLAS_STATUS_ALLOC_FAILED;
+    if (!device_accessible(A) || !device_accessible(B) || !device_accessible(C))
+        return SSBLAS_STATUS_INVALID_VALUE;
     std::vector<char> image(bytes, 0);
     memcpy(image.data() + off_groups, groups.data(), groups.size() * sizeof(GGroup));
     memcpy(image.data() + off_insts, insts.data(), insts.size() * sizeof(GInst));

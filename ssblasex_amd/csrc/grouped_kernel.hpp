@@ -24,6 +24,7 @@
 #include "gemm_kernels.hpp"
 #include "grouped.hpp"
 #include "launch.hpp"
+#include "occupancy.hpp"
 
 namespace ssb {
 
@@ -463,15 +464,7 @@ hipError_t run_grouped(const GroupedDevice& d, hipStream_t st)
 {
     if (d.nitems + d.nwitems <= 0) return hipSuccess;
     auto kern = grouped_gemm<FA, FB>;
-    static int occ[16] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
-    if (occ[dev] == 0) {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, 0) != hipSuccess || nb < 1) nb = 1;
-        occ[dev] = nb;
-    }
-    const long slots = (long)occ[dev] * device_cu_count();
+    const long slots = (long)kernel_occupancy(kern, 256) * device_cu_count();
     const long want = (long)d.nitems + ((long)d.nwitems + 3) / 4;  // four waves take wave items at once
     const long grid = want < slots ? want : slots;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, d);

@@ -7,6 +7,7 @@
 #include "gemm_dma_kc.hpp"
 #include "gemm_kernels.hpp"
 #include "launch.hpp"
+#include "occupancy.hpp"
 
 namespace ssb {
 
@@ -50,23 +51,6 @@ SSB_CFG(CFG_TILE_128_W8_S2, 128, 128, 4, 2, (AR::ID == ARITH_F32 ? 2 : 1), 2)
 SSB_CFG(CFG_TILE_256x128_S2, 256, 128, 4, 2, 1, 2)
 #undef SSB_CFG
 
-template <class K>
-struct OccCache {
-    static int get(K kernel, int threads)
-    {
-        static int cached[16] = {0};
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
-        if (cached[dev] == 0) {
-            int nb = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, threads, 0) != hipSuccess || nb < 1)
-                nb = 1;
-            cached[dev] = nb;
-        }
-        return cached[dev];
-    }
-};
-
 // Persistent grid: every tile gets its own workgroup when they all fit at
 // once; otherwise as many workgroups as are resident, a multiple of 8 so
 // that each workgroup keeps one XCD's share of the work ids.
@@ -95,7 +79,7 @@ hipError_t run_tile(KParams p, hipStream_t st)
     p.tiles_m = (p.m + CS::BM - 1) / CS::BM;
     p.tiles_n = (p.n + CS::BN - 1) / CS::BN;
     p.total = (long)p.batch * p.tiles_m * p.tiles_n;
-    int occ = OccCache<decltype(kern)>::get(kern, NT);
+    int occ = kernel_occupancy(kern, NT);
     long grid = persistent_grid(p.total, occ);
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), 0, st, p);
     return hipGetLastError();
@@ -115,12 +99,12 @@ hipError_t run_dma(KParams p, hipStream_t st)
         const bool whole = p.m % BM == 0 && p.n % BN == 0 && p.k % BK == 0 && p.lda % 16 == 0 && p.ldb % 16 == 0;
         if (whole && getenv("SSBLAS_NO_STREAM") == nullptr) {
             auto kern = dma_stream_gemm_i8<AKC, BKC, BM, BN, BK, WM, WN, S>;
-            int occ = OccCache<decltype(kern)>::get(kern, NT);
+            int occ = kernel_occupancy(kern, NT);
             long grid = persistent_grid(p.total, occ);
             hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), 0, st, p);
         } else {
             auto kern = dma_gemm_i8<AKC, BKC, BM, BN, BK, WM, WN, S>;
-            int occ = OccCache<decltype(kern)>::get(kern, NT);
+            int occ = kernel_occupancy(kern, NT);
             long grid = persistent_grid(p.total, occ);
             hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), 0, st, p);
         }
@@ -149,7 +133,7 @@ hipError_t run_kc(KParams p, hipStream_t st)
         p.tiles_m = p.m / BM;
         p.tiles_n = p.n / BN;
         p.total = (long)p.batch * p.tiles_m * p.tiles_n;
-        int occ = OccCache<decltype(kern)>::get(kern, NT);
+        int occ = kernel_occupancy(kern, NT);
         long grid = persistent_grid(p.total, occ);
         hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(NT), 0, st, p);
         return hipGetLastError();
@@ -163,7 +147,7 @@ hipError_t run_direct(KParams p, hipStream_t st)
     p.tiles_m = (p.m + W - 1) / W;
     p.tiles_n = (p.n + W - 1) / W;
     p.total = (long)p.batch * p.tiles_m * p.tiles_n;
-    int occ = OccCache<decltype(kern)>::get(kern, 256);
+    int occ = kernel_occupancy(kern, 256);
     long blocks = (p.total + 3) / 4;
     long grid = persistent_grid(blocks, occ);
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, p);

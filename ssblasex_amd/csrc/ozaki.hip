@@ -26,7 +26,9 @@
 #include <stdlib.h>
 
 #include "gemm_kernels.hpp"
+#include "api_internal.hpp"
 #include "launch.hpp"
+#include "occupancy.hpp"
 
 namespace ssb {
 
@@ -557,17 +559,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
 template <int S, bool FULL>
 static hipError_t run_ozaki_small_k(const OzParams& p, hipStream_t st)
 {
-    static int occ[16] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) dev = 0;
-    if (occ[dev] == 0) {
-        int nb = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ozaki_small<S, FULL>, 256, 0) != hipSuccess || nb < 1)
-            nb = 1;
-        occ[dev] = nb;
-    }
+    const int occ = kernel_occupancy(ozaki_small<S, FULL>, 256);
     const long blocks = ((long)p.batch + 3) / 4;
-    long grid = (long)occ[dev] * device_cu_count();
+    long grid = (long)occ * device_cu_count();
     if (const char* e = getenv("SSBLAS_MAX_GRID")) {
         const long cap = atol(e);
         if (cap >= 1 && cap < grid) grid = cap;
@@ -608,10 +602,7 @@ static hipError_t run_ozaki_gemm(OzParams p, hipStream_t st)
     p.tiles_m = (p.m + BM - 1) / BM;
     p.tiles_n = (p.n + BN - 1) / BN;
     p.total = (long)p.batch * p.tiles_m * p.tiles_n;
-    static int occ = 0;
-    if (occ == 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, 256, 0) != hipSuccess || occ < 1) occ = 1;
-    }
+    const int occ = kernel_occupancy(kern, 256);
     long slots = (long)occ * device_cu_count();
     long grid = p.total <= slots ? p.total : (slots / 8) * 8;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, p);
@@ -638,7 +629,7 @@ static hipError_t prep_operand(const void* const* X, long ld, int nrows, int nco
 hipError_t ozaki_run(const GemmProblem& g, int s, void* (*ws_get)(size_t, hipError_t*), hipStream_t st)
 {
     const int m = g.m, n = g.n, k = g.k;
-    if (m <= 32 && n <= 32 && k <= 32 && s >= 3 && s <= 9 && getenv("SSBLAS_OZAKI_NO_FUSED") == nullptr) {
+    if (ozaki_fused_shape(g) && s >= 3 && s <= 9) {
         // fused single-wave form: no workspace
         OzParams p{};
         p.A = g.A;
@@ -659,18 +650,13 @@ hipError_t ozaki_run(const GemmProblem& g, int s, void* (*ws_get)(size_t, hipErr
         p.beta0 = g.beta0 ? 1 : 0;
         return ozaki_small_dispatch(p, st);
     }
-    // slice storage keeps the operand's orientation, leading dimension
-    // rounded up to 16 bytes
+    const OzakiLayout L = ozaki_layout(g, s);
+    hipError_t err = hipSuccess;
+    char* ws = static_cast<char*>(ws_get(L.total, &err));
+    if (ws == nullptr) return err == hipSuccess ? hipErrorOutOfMemory : err;
     const long ra = g.ta ? k : m, ca = g.ta ? m : k;  // storage of A
     const long rb = g.tb ? n : k, cb = g.tb ? k : n;  // storage of B
-    const long ld_sa = (ra + 15) / 16 * 16, ld_sb = (rb + 15) / 16 * 16;
-    const size_t bytes_a = (size_t)g.batch * s * ld_sa * ca;
-    const size_t bytes_b = (size_t)g.batch * s * ld_sb * cb;
-    const size_t bytes_e = (size_t)g.batch * (m + n) * sizeof(int32_t);
-    const size_t total = ((bytes_a + 255) / 256 + (bytes_b + 255) / 256) * 256 + bytes_e;
-    hipError_t err = hipSuccess;
-    char* ws = static_cast<char*>(ws_get(total, &err));
-    if (ws == nullptr) return err == hipSuccess ? hipErrorOutOfMemory : err;
+    const long ld_sa = L.ld_sa, ld_sb = L.ld_sb;
 
     OzParams p;
     p.A = g.A;
@@ -687,8 +673,8 @@ hipError_t ozaki_run(const GemmProblem& g, int s, void* (*ws_get)(size_t, hipErr
     p.ta = g.ta;
     p.tb = g.tb;
     p.wa = reinterpret_cast<int8_t*>(ws);
-    p.wb = reinterpret_cast<int8_t*>(ws + (bytes_a + 255) / 256 * 256);
-    p.ea = reinterpret_cast<int32_t*>(ws + ((bytes_a + 255) / 256 + (bytes_b + 255) / 256) * 256);
+    p.wb = reinterpret_cast<int8_t*>(ws + L.off_b);
+    p.ea = reinterpret_cast<int32_t*>(ws + L.off_e);
     p.eb = p.ea + (size_t)g.batch * m;
     p.ld_sa = ld_sa;
     p.cols_sa = ca;

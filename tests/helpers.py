@@ -205,3 +205,134 @@ def run_gpu(b: Batch, compute_type=0, entry="sync", dev: DeviceBatch | None = No
 
         st = S.ssblasGemmBatchedExAsync(*args, stream=torch.cuda.current_stream())
     return st, d.results()
+
+
+# ---------------------------------------------------------------------------
+# benchmark-sized batches: one buffer per operand, instance i at a fixed stride
+# ---------------------------------------------------------------------------
+class FlatBatch:
+    """A batch of `batch` instances held in one (batch, elems) host array per
+    operand, filled by vectorised numpy draws (seeded).  Padding rows of C
+    hold a sentinel (3) so that writes outside the m x n block show up.
+    Instance lists (A, B, C0) are row views, usable wherever a Batch is."""
+
+    def __init__(self, type_, transa, transb, m, n, k, batch, *, alpha=1.0, beta=0.0, pad=0, seed=0,
+                 dist="uniform"):
+        rng = np.random.default_rng(seed)
+        _, _, abt, ct = TYPES[type_]
+        self.type, self.transa, self.transb, self.m, self.n, self.k = type_, transa, transb, m, n, k
+        ra, ca = oracle.op_dims(transa, m, k)
+        rb, cb = oracle.op_dims(transb, k, n)
+        self.lda, self.ldb, self.ldc = ra + pad, rb + pad, m + pad
+        if type_ == "i8":
+            alpha, beta = int(alpha), int(beta)
+        self.alpha, self.beta = alpha, beta
+
+        def draw(rows, cols, ld, kind, dtype):
+            x = np.full((batch, cols, ld), 7 if dtype != ct else 3, dtype=dtype)
+            if kind == "i8":
+                x[:, :, :rows] = rng.integers(-128, 128, size=(batch, cols, rows), dtype=np.int8)
+            elif kind == "i32c":
+                x[:, :, :rows] = rng.integers(-1000, 1001, size=(batch, cols, rows), dtype=np.int32)
+            elif kind == "normal":
+                x[:, :, :rows] = rng.standard_normal((batch, cols, rows), dtype=np.float64 if dtype == np.float64
+                                                     else np.float32)
+            elif kind == "wide":
+                x[:, :, :rows] = rng.uniform(-1, 1, (batch, cols, rows)) * np.exp2(
+                    rng.integers(-8, 9, (batch, cols, rows)))
+            else:
+                x[:, :, :rows] = rng.uniform(-1, 1, (batch, cols, rows))
+            return x.reshape(batch, cols * ld)
+
+        kind_ab = "i8" if type_ == "i8" else dist
+        self.Aflat = draw(ra, ca, self.lda, kind_ab, abt)
+        self.Bflat = draw(rb, cb, self.ldb, kind_ab, abt)
+        self.C0flat = draw(m, n, self.ldc, "i32c" if type_ == "i8" else dist, ct)
+        self.A = list(self.Aflat)
+        self.B = list(self.Bflat)
+        self.C0 = list(self.C0flat)
+
+    batch = property(lambda self: len(self.C0))
+    ab_code = property(lambda self: TYPES[self.type][0])
+    c_code = property(lambda self: TYPES[self.type][1])
+
+    def dims_a(self):
+        return oracle.op_dims(self.transa, self.m, self.k)
+
+    def dims_b(self):
+        return oracle.op_dims(self.transb, self.k, self.n)
+
+    def sample(self, idx) -> Batch:
+        """The instances `idx` as an ordinary Batch (copies)."""
+        b = Batch(self.type, self.transa, self.transb, self.m, self.n, self.k, self.lda, self.ldb, self.ldc,
+                  self.alpha, self.beta)
+        b.A = [self.Aflat[i].copy() for i in idx]
+        b.B = [self.Bflat[i].copy() for i in idx]
+        b.C0 = [self.C0flat[i].copy() for i in idx]
+        return b
+
+    def cview(self, flat):
+        """(batch, m, n) view of the m x n blocks of a (batch, ldc*n) array."""
+        return flat.reshape(self.batch, self.n, self.ldc)[:, :, :self.m].transpose(0, 2, 1)
+
+    def norms(self):
+        """Per-instance normwise-error scale |alpha| ||op(A)||_F ||op(B)||_F + |beta| ||C0||_F."""
+        ra, ca = self.dims_a()
+        rb, cb = self.dims_b()
+        na = np.sqrt((self.Aflat.reshape(self.batch, ca, self.lda)[:, :, :ra].astype(np.float64) ** 2).sum(axis=(1, 2)))
+        nb = np.sqrt((self.Bflat.reshape(self.batch, cb, self.ldb)[:, :, :rb].astype(np.float64) ** 2).sum(axis=(1, 2)))
+        s = abs(self.alpha) * na * nb
+        if self.beta != 0:
+            s = s + abs(self.beta) * np.sqrt((self.cview(self.C0flat).astype(np.float64) ** 2).sum(axis=(1, 2)))
+        return s
+
+    def errors(self, Cflat, hi, lo):
+        """Per-instance ||C - (hi + lo)||_F / scale, the difference taken in
+        double-double order; hi, lo are (batch, m, n)."""
+        d = (self.cview(Cflat).astype(np.float64) - hi) - lo
+        return np.sqrt((d * d).sum(axis=(1, 2))) / self.norms()
+
+    def padding_untouched(self, Cflat):
+        x = Cflat.reshape(self.batch, self.n, self.ldc)[:, :, self.m:]
+        y = self.C0flat.reshape(self.batch, self.n, self.ldc)[:, :, self.m:]
+        return np.array_equal(x, y)
+
+
+class FlatDeviceBatch:
+    """Device copies of a FlatBatch: one allocation per operand, pointer
+    arrays at the instance stride (three copies in all, however large the
+    batch)."""
+
+    def __init__(self, fb: FlatBatch, device="cuda"):
+        import torch
+
+        from ssblasex_amd import pointer_array_strided
+
+        self.fb = fb
+        self.A = torch.from_numpy(fb.Aflat).to(device)
+        self.B = torch.from_numpy(fb.Bflat).to(device)
+        self.C = torch.from_numpy(fb.C0flat).to(device)
+        self.pA = pointer_array_strided(self.A, fb.batch, self.A.shape[1])
+        self.pB = pointer_array_strided(self.B, fb.batch, self.B.shape[1])
+        self.pC = pointer_array_strided(self.C, fb.batch, self.C.shape[1])
+
+    def reset_c(self):
+        import torch
+
+        self.C.copy_(torch.from_numpy(self.fb.C0flat))
+
+    def run(self, compute_type=0):
+        """One synchronous ssblasGemmBatchedEx over the whole batch from the
+        initial C; returns (status, host copy of C as (batch, ldc*n))."""
+        import torch
+
+        import ssblasex_amd as S
+
+        fb = self.fb
+        self.reset_c()
+        torch.cuda.synchronize()
+        st = S.ssblasGemmBatchedEx(fb.transa, fb.transb, fb.m, fb.n, fb.k, fb.alpha, self.pA, fb.ab_code, fb.lda,
+                                   self.pB, fb.ab_code, fb.ldb, fb.beta, self.pC, fb.c_code, fb.ldc, fb.batch,
+                                   compute_type)
+        torch.cuda.synchronize()
+        return st, self.C.cpu().numpy()

@@ -204,6 +204,39 @@ def test_kc_dma_tiles(cfg, form, ct, factor, grid):
         os.environ.pop("SSBLAS_MAX_GRID", None)
 
 
+SPLIT_FORMS = [("bf16x6", 5, 4.0), ("bf16x3", 4, None)]
+
+
+@pytest.mark.parametrize("cfg", ["tile256x128", "tile256x128s2", "tile128w8", "tile128", "tile64"])
+@pytest.mark.parametrize("form,ct,factor", SPLIT_FORMS, ids=[f[0] for f in SPLIT_FORMS])
+@pytest.mark.parametrize("ta,tb", OPS)
+def test_split_forms_forced_tiles(cfg, form, ct, factor, ta, tb):
+    """The BF16 split forms on the register-staged tiles the tuned table
+    sends large fp32 batches to (256x128 with its swizzled unpadded bf16 LDS
+    rows), forced: whole tiles at beta 0, ragged tiles with a K tail and
+    padded leading dimensions at beta != 0, and a grid capped so that every
+    workgroup walks several tiles."""
+    os.environ["SSBLAS_TILE_CONFIG"] = cfg
+    bar = (lambda e, r: e <= 2.0 ** -14) if factor is None else (lambda e, r: e <= fp_bar("f32", r, factor))
+    cases = [dict(m=256, n=256, k=256, batch=2, alpha=1.0, beta=0.0, pad=(0, 0, 0), seed=81),
+             dict(m=300, n=200, k=150, batch=2, alpha=1.5, beta=-0.5, pad=(3, 1, 2), seed=82),
+             dict(m=512, n=384, k=96, batch=3, alpha=-0.75, beta=2.0, pad=(0, 0, 0), seed=83)]
+    try:
+        for i, c in enumerate(cases):
+            if i == 2:
+                os.environ["SSBLAS_MAX_GRID"] = "5"
+            b = make_batch("f32", ta, tb, c["m"], c["n"], c["k"], c["batch"], alpha=c["alpha"], beta=c["beta"],
+                           pad=c["pad"], seed=c["seed"], dist="normal")
+            st, C = run_gpu(b, compute_type=ct)
+            assert st == 0
+            assert padding_untouched(b, C)
+            _, R = run_oracle(b, NTHREADS)
+            e, r = normwise(b, C), normwise(b, R)
+            assert bar(e, r), (c, e, r)
+    finally:
+        os.environ.pop("SSBLAS_MAX_GRID", None)
+
+
 @pytest.mark.parametrize("t", ["i8", "f32", "f64"])
 @pytest.mark.parametrize("ta,tb", OPS)
 def test_unaligned_pointers_and_odd_ld(t, ta, tb):

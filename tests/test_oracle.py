@@ -225,3 +225,45 @@ def test_planted_mistake_is_caught():
     st2, C2 = run_oracle(bt)
     assert not np.array_equal(C[0], C2[0])
     assert np.array_equal(oracle.colmajor(C[0], 6, 6, 6), numpy_exact(b, 0))
+
+
+@pytest.mark.parametrize("t", ["f32", "f64", "i8"])
+@pytest.mark.parametrize("beta", [0.0, -0.75])
+def test_dd_batched_equals_dd(t, beta):
+    """oracle_gemm_dd_batched is oracle_gemm_dd per instance, bit for bit
+    (the batched form only changes the parallel loop)."""
+    for ta, tb in OPS:
+        b = make_batch(t, ta, tb, 9, 7, 13, 5, alpha=1.25 if t != "i8" else 3, beta=beta if t != "i8" else int(beta * 4),
+                       pad=(1, 2, 3), seed=77)
+        hi, lo = oracle.gemm_dd_batched(ta, tb, b.m, b.n, b.k, b.alpha, b.A, b.ab_code, b.lda, b.B, b.ldb, b.beta,
+                                        b.C0, b.ldc, nthreads=3)
+        for i in range(b.batch):
+            h1, l1 = oracle.gemm_dd(ta, tb, b.m, b.n, b.k, b.alpha, b.A[i], b.ab_code, b.lda, b.B[i], b.ldb, b.beta,
+                                    b.C0[i], b.ldc)
+            assert np.array_equal(hi[i], h1) and np.array_equal(lo[i], l1)
+
+
+@pytest.mark.skipif(oracle.find_openblas() is None, reason="scipy's OpenBLAS not installed")
+@pytest.mark.parametrize("t", ["f32", "f64"])
+def test_blas_baseline_matches_numpy(t):
+    """The example's cblas loop (scipy's OpenBLAS, opened at run time):
+    trans flags, leading dimensions, alpha and beta map as in the call, so
+    on integer-valued inputs (every sum exact) it equals exact arithmetic;
+    a transposed operand planted in the call must not."""
+    lib = oracle.find_openblas()
+    for ta, tb in OPS:
+        b = make_batch(t, ta, tb, 11, 6, 9, 3, alpha=2.0, beta=-3.0, pad=(2, 1, 3), seed=91)
+        for arr in b.A + b.B + b.C0:
+            arr[:] = np.round(arr * 8)
+        C = [c.copy() for c in b.C0]
+        assert oracle.blas_baseline(lib, b.ab_code, ta, tb, b.m, b.n, b.k, b.alpha, b.A, b.lda, b.B, b.ldb, b.beta,
+                                    C, b.ldc, 2) == 0
+        assert padding_untouched(b, C)
+        for i in range(b.batch):
+            assert np.array_equal(oracle.colmajor(C[i], b.m, b.n, b.ldc), numpy_exact(b, i))
+    b = make_batch(t, 0, 0, 6, 6, 6, 1, alpha=1.0, beta=0.0, seed=92)
+    for arr in b.A + b.B:
+        arr[:] = np.round(arr * 8)
+    C = [c.copy() for c in b.C0]
+    oracle.blas_baseline(lib, b.ab_code, 0, 1, 6, 6, 6, 1.0, b.A, b.lda, b.B, b.ldb, 0.0, C, b.ldc, 1)
+    assert not np.array_equal(oracle.colmajor(C[0], 6, 6, 6), numpy_exact(b, 0))
