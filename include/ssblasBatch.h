@@ -135,6 +135,43 @@ ssblasStatus_t ssblasGemmBatchedExAsync(
     ssblasStream_t       stream);
 
 /*
+ * ssblasGemmBatchedExAsync for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, status
+ * table and results as ssblasGemmBatchedExAsync; it calls no hipMalloc or
+ * hipFree.  The device workspace of the Ozaki form comes from a fixed
+ * scratch array of SSBLAS_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; a call that needs more returns SSBLAS_STATUS_ALLOC_FAILED
+ * before any device work (the plain and BF16 forms and the fused Ozaki form
+ * for m, n, k <= 32 need none).  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (other streams, other threads) race
+ * on it; calls on one stream are ordered.
+ */
+#define SSBLAS_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+
+ssblasStatus_t ssblasGemmBatchedExNoAlloc(
+    ssblasOperation_t    transa,
+    ssblasOperation_t    transb,
+    SSINT                m,
+    SSINT                n,
+    SSINT                k,
+    const void*          alpha,
+    const void* const    A[],
+    ssblasDataType_t     Atype,
+    const SSINT          lda,
+    const void* const    B[],
+    ssblasDataType_t     Btype,
+    const SSINT          ldb,
+    const void*          beta,
+    void* const          C[],
+    ssblasDataType_t     Ctype,
+    const SSINT          ldc,
+    const SSINT          batchCount,
+    ssblasComputeType_t  computeType,
+    ssblasGemmAlgo_t     algo,
+    ssblasStream_t       stream);
+
+/*
  * Grouped (variable-size) batched GEMM (no counterpart in the reference).
  *
  * Instances are numbered 0 .. total-1, total = sum(group_size[g]); group g

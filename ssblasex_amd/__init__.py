@@ -160,6 +160,20 @@ def ssblasGemmBatchedExAsync(transa, transb, m, n, k, alpha, A, Atype, lda, B, B
     return Status(_lib.load().ssblasGemmBatchedExAsync(*args, s))
 
 
+def ssblasGemmBatchedExNoAlloc(transa, transb, m, n, k, alpha, A, Atype, lda, B, Btype, ldb, beta, C,
+                               Ctype, ldc, batchCount, computeType=ComputeType.DEFAULT_TYPE,
+                               algo=GemmAlgo.DEFAULT, stream=0) -> Status:
+    """ssblasGemmBatchedExAsync that allocates no device memory (the Ozaki
+    workspace comes from a fixed per-GPU scratch array)."""
+    args, keep = _gemm_args(transa, transb, m, n, k, alpha, A, Atype, lda, B, Btype, ldb, beta, C, Ctype,
+                            ldc, batchCount, computeType, algo)
+    s = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream or 0)
+    return Status(_lib.load().ssblasGemmBatchedExNoAlloc(*args, s))
+
+
+NOALLOC_SCRATCH_BYTES = 64 << 20  # SSBLAS_NOALLOC_SCRATCH_BYTES of include/ssblasBatch.h
+
+
 def kernel_config(transa, transb, m, n, k, Atype, Btype, Ctype, lda, ldb, ldc, batchCount,
                   computeType=0) -> str:
     """Name of the kernel (tile config or Ozaki family) the call would run."""
@@ -200,6 +214,6 @@ from .grouped import GroupedPlan, ssblasGemmGroupedBatchedEx  # noqa: E402,F401
 
 __all__ = [
     "Status", "Operation", "DataType", "ComputeType", "GemmAlgo", "scale_ctype",
-    "ssblasGemmBatchedEx", "ssblasGemmBatchedEx_64", "ssblasGemmBatchedExAsync",
+    "ssblasGemmBatchedEx", "ssblasGemmBatchedEx_64", "ssblasGemmBatchedExAsync", "ssblasGemmBatchedExNoAlloc",
     "ssblasGemmGroupedBatchedEx", "GroupedPlan", "pointer_array", "pointer_array_strided", "kernel_config", "version",
 ]

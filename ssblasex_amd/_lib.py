@@ -40,6 +40,7 @@ SIGNATURES = {
     "ssblasGemmBatchedEx": (_c_int, _GEMM_ARGS),
     "ssblasGemmBatchedEx_64": (_c_int, _GEMM_ARGS),
     "ssblasGemmBatchedExAsync": (_c_int, _GEMM_ARGS + [_vp]),
+    "ssblasGemmBatchedExNoAlloc": (_c_int, _GEMM_ARGS + [_vp]),
     "ssblasGroupedPlanCreate": (_c_int, [_vp] + _GROUPED_ARGS),
     "ssblasGroupedPlanExecute": (_c_int, [_vp, _vp]),
     "ssblasGroupedPlanGraphLaunch": (_c_int, [_vp, _vp]),

@@ -15,6 +15,7 @@ from __future__ import annotations
 import argparse
 import concurrent.futures as cf
 import os
+import re
 import subprocess
 import sys
 from pathlib import Path
@@ -48,6 +49,22 @@ def _headers() -> list[Path]:
     return sorted(CSRC.glob("*.hpp")) + sorted((ROOT / "include").glob("*.h"))
 
 
+_INCLUDE = re.compile(r'^\s*#\s*include\s+"([^"]+)"', re.M)
+
+
+def _deps(src: Path) -> list[Path]:
+    """src and every file it includes with #include "..." (transitively)."""
+    seen, todo = [], [src]
+    while todo:
+        f = todo.pop()
+        if f in seen or not f.exists():
+            continue
+        seen.append(f)
+        for name in _INCLUDE.findall(f.read_text()):
+            todo.append((f.parent / name).resolve())
+    return seen
+
+
 def _needs(obj: Path, deps: list[Path]) -> bool:
     if not obj.exists():
         return True
@@ -57,7 +74,7 @@ def _needs(obj: Path, deps: list[Path]) -> bool:
 
 def _compile(src: Path, force: bool) -> tuple[Path, str | None]:
     obj = OBJ / (src.name + ".o")
-    if not force and not _needs(obj, [src] + _headers()):
+    if not force and not _needs(obj, _deps(src)):
         return obj, None
     if src.suffix == ".hip":
         cmd = [str(ROCM / "bin" / "hipcc"), *HIP_FLAGS, "-c", str(src), "-o", str(obj)]

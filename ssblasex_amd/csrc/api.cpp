@@ -209,16 +209,16 @@ void fill_scalars(TypeTriple tt, const void* alpha, const void* beta, GemmProble
 // ---------------------------------------------------------------------------
 // The batched call
 // ---------------------------------------------------------------------------
-ssblasStatus_t gemm_batched(ssblasOperation_t transa, ssblasOperation_t transb, long m, long n, long k,
+ssblasStatus_t prepare_call(ssblasOperation_t transa, ssblasOperation_t transb, long m, long n, long k,
                             const void* alpha, const void* const A[], ssblasDataType_t Atype, long lda,
                             const void* const B[], ssblasDataType_t Btype, long ldb, const void* beta,
                             void* const C[], ssblasDataType_t Ctype, long ldc, long batchCount,
-                            ssblasComputeType_t computeType, ssblasGemmAlgo_t algo, hipStream_t stream,
-                            bool sync)
+                            ssblasComputeType_t computeType, ssblasGemmAlgo_t algo, size_t ws_cap,
+                            PreparedCall* pc)
 {
     if (getenv("SSBLAS_GEMMBATCHEDEX_DEBUG") != nullptr)
         debug_dump(transa, transb, m, n, k, Atype, lda, Btype, ldb, Ctype, ldc, batchCount, computeType, algo);
-
+    pc->run = false;
     TypeTriple tt = classify_types(Atype, Btype, Ctype);
     if (tt == TT_NONE) return SSBLAS_STATUS_INVALID_VALUE;
     ssblasStatus_t st = check_shape(transa, transb, m, n, k, lda, ldb, ldc, batchCount);
@@ -229,7 +229,8 @@ ssblasStatus_t gemm_batched(ssblasOperation_t transa, ssblasOperation_t transb, 
     if (alpha == nullptr || beta == nullptr) return SSBLAS_STATUS_INVALID_VALUE;
     if (k == 0) return SSBLAS_STATUS_SUCCESS;  // no K block: C unchanged (reference ..._BMKN.hpp:200)
 
-    GemmProblem g;
+    GemmProblem& g = pc->g;
+    g = GemmProblem{};
     g.ta = transa == SSBLAS_OP_T;
     g.tb = transb == SSBLAS_OP_T;
     g.m = (int)m;
@@ -243,26 +244,45 @@ ssblasStatus_t gemm_batched(ssblasOperation_t transa, ssblasOperation_t transb, 
     g.B = B;
     g.C = C;
     fill_scalars(tt, alpha, beta, g);
-    // Library-owned workspace (the Ozaki form only) beyond the caller's cap.
-    if (form == FORM_OZAKI && ozaki_workspace_bytes(g, slices) > workspace_limit())
-        return SSBLAS_STATUS_ALLOC_FAILED;
+    g.arith = form == FORM_I8 ? K_I8 : form == FORM_F32 ? K_F32 : form == FORM_F64 ? K_F64
+            : form == FORM_BF16X3 ? K_BF16X3 : form == FORM_BF16X6 ? K_BF16X6 : K_F64;
+    pc->form = form;
+    pc->slices = slices;
+    // Device workspace (the Ozaki form only) beyond the cap.
+    if (form == FORM_OZAKI && ozaki_workspace_bytes(g, slices) > ws_cap) return SSBLAS_STATUS_ALLOC_FAILED;
     if (!device_accessible(A) || !device_accessible(B) || !device_accessible(C))
         return SSBLAS_STATUS_INVALID_VALUE;
+    pc->run = true;
+    return SSBLAS_STATUS_SUCCESS;
+}
 
+ssblasStatus_t run_prepared(const PreparedCall& pc, hipStream_t stream, WsGet ws_get)
+{
     hipError_t e;
-    if (form == FORM_OZAKI) {
-        e = launch_ozaki(g, slices, stream, &st);
+    if (pc.form == FORM_OZAKI) {
+        ssblasStatus_t st = SSBLAS_STATUS_SUCCESS;
+        e = launch_ozaki(pc.g, pc.slices, stream, ws_get, &st);
         if (st != SSBLAS_STATUS_SUCCESS) return st;
     } else {
-        g.arith = form == FORM_I8 ? K_I8 : form == FORM_F32 ? K_F32 : form == FORM_F64 ? K_F64
-                : form == FORM_BF16X3 ? K_BF16X3 : K_BF16X6;
-        e = launch_gemm(g, pick_config(g), stream);
+        e = launch_gemm(pc.g, pick_config(pc.g), stream);
     }
-    if (e != hipSuccess) return hip_status(e);
-    if (sync) {
-        e = hipStreamSynchronize(stream);
-        if (e != hipSuccess) return hip_status(e);
-    }
+    return hip_status(e);
+}
+
+ssblasStatus_t gemm_batched(ssblasOperation_t transa, ssblasOperation_t transb, long m, long n, long k,
+                            const void* alpha, const void* const A[], ssblasDataType_t Atype, long lda,
+                            const void* const B[], ssblasDataType_t Btype, long ldb, const void* beta,
+                            void* const C[], ssblasDataType_t Ctype, long ldc, long batchCount,
+                            ssblasComputeType_t computeType, ssblasGemmAlgo_t algo, hipStream_t stream,
+                            bool sync)
+{
+    PreparedCall pc;
+    ssblasStatus_t st = prepare_call(transa, transb, m, n, k, alpha, A, Atype, lda, B, Btype, ldb, beta, C, Ctype,
+                                     ldc, batchCount, computeType, algo, workspace_limit(), &pc);
+    if (st != SSBLAS_STATUS_SUCCESS || !pc.run) return st;
+    st = run_prepared(pc, stream, nullptr);
+    if (st != SSBLAS_STATUS_SUCCESS) return st;
+    if (sync) return hip_status(hipStreamSynchronize(stream));
     return SSBLAS_STATUS_SUCCESS;
 }
 

@@ -27,12 +27,36 @@ int pick_config(const GemmProblem& g);
 // Autotuned config for the problem's size bucket, -1 when none.
 int tuned_config(const GemmProblem& g);
 
+// A device workspace provider: returns a buffer of at least the requested
+// bytes (or nullptr with *err set).
+using WsGet = void* (*)(size_t bytes, hipError_t* err);
+
+// One batched call after the host-side checks.
+struct PreparedCall {
+    GemmProblem g;
+    int form;    // Form
+    int slices;  // Ozaki slice count named by the compute type (0: automatic)
+    bool run;    // false: nothing to compute (k == 0)
+};
+// The status table of the batched entry points, decided on the host (the
+// reference's order of checks); a call whose Ozaki workspace would exceed
+// ws_cap gets SSBLAS_STATUS_ALLOC_FAILED.
+ssblasStatus_t prepare_call(ssblasOperation_t transa, ssblasOperation_t transb, long m, long n, long k,
+                            const void* alpha, const void* const A[], ssblasDataType_t Atype, long lda,
+                            const void* const B[], ssblasDataType_t Btype, long ldb, const void* beta,
+                            void* const C[], ssblasDataType_t Ctype, long ldc, long batchCount,
+                            ssblasComputeType_t computeType, ssblasGemmAlgo_t algo, size_t ws_cap,
+                            PreparedCall* pc);
+// Enqueue a prepared call on `stream`; ws_get == nullptr: the library-owned
+// workspace of (device, stream).
+ssblasStatus_t run_prepared(const PreparedCall& pc, hipStream_t stream, WsGet ws_get);
+
 // Ozaki-scheme fp64 (slices == 0: pick automatically).  Sets *st to a
 // non-SUCCESS status for failures that are not HIP errors.
-hipError_t launch_ozaki(const GemmProblem& g, int slices, hipStream_t stream, ssblasStatus_t* st);
+hipError_t launch_ozaki(const GemmProblem& g, int slices, hipStream_t stream, WsGet ws_get, ssblasStatus_t* st);
 // Device side of the Ozaki form (ozaki.hip); ws_get returns a device buffer
 // of at least the requested size owned by the caller's stream.
-hipError_t ozaki_run(const GemmProblem& g, int s, void* (*ws_get)(size_t, hipError_t*), hipStream_t st);
+hipError_t ozaki_run(const GemmProblem& g, int s, WsGet ws_get, hipStream_t st);
 // Workspace of the general Ozaki path: the int8 slices of op(A) and op(B)
 // (each kept in its operand's storage orientation, leading dimension rounded
 // up to 16 bytes), then the int32 exponent vectors.

@@ -626,7 +626,7 @@ static hipError_t prep_operand(const void* const* X, long ld, int nrows, int nco
     return hipGetLastError();
 }
 
-hipError_t ozaki_run(const GemmProblem& g, int s, void* (*ws_get)(size_t, hipError_t*), hipStream_t st)
+hipError_t ozaki_run(const GemmProblem& g, int s, WsGet ws_get, hipStream_t st)
 {
     const int m = g.m, n = g.n, k = g.k;
     if (ozaki_fused_shape(g) && s >= 3 && s <= 9) {

@@ -85,12 +85,15 @@ const char* ozaki_kernel_name(const GemmProblem& g)
     return ozaki_fused_shape(g) ? "ozaki_small" : "ozaki_gemm";
 }
 
-hipError_t launch_ozaki(const GemmProblem& g, int slices, hipStream_t stream, ssblasStatus_t* st)
+hipError_t launch_ozaki(const GemmProblem& g, int slices, hipStream_t stream, WsGet ws_get, ssblasStatus_t* st)
 {
     *st = SSBLAS_STATUS_SUCCESS;
     const int s = slices > 0 ? slices : OZAKI_DEFAULT_SLICES;
-    t_stream = stream;
-    hipError_t e = ozaki_run(g, s, get_ws, stream);
+    if (ws_get == nullptr) {
+        t_stream = stream;
+        ws_get = get_ws;
+    }
+    hipError_t e = ozaki_run(g, s, ws_get, stream);
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) *st = SSBLAS_STATUS_ALLOC_FAILED;
     return e;
 }

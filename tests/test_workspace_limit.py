@@ -72,3 +72,18 @@ def test_grouped_plan_over_limit(limit):
     limit("1G")
     st, plan = S.GroupedPlan.create(*args)
     assert plan is None and st == S.Status.INVALID_VALUE
+
+
+def test_noalloc_entry_refuses_beyond_scratch(limit):
+    """ssblasGemmBatchedExNoAlloc takes the Ozaki workspace from a fixed
+    per-GPU scratch array of SSBLAS_NOALLOC_SCRATCH_BYTES (64 MiB): a call
+    needing more is refused on the host, whatever SSBLAS_WORKSPACE_LIMIT says."""
+    limit("1G")
+
+    def call(batch, ct=0x109):
+        return S.ssblasGemmBatchedExNoAlloc(0, 0, 256, 256, 256, 1.0, ADDR, R_64F, 256, ADDR, R_64F, 256, 0.0, ADDR,
+                                            R_64F, 256, batch, ct)
+
+    assert call(64) == S.Status.ALLOC_FAILED   # 72 MiB of slices
+    assert call(48) == S.Status.INVALID_VALUE  # 54 MiB: fits, then the host pointer arrays are refused
+    assert call(64, ct=0) == S.Status.INVALID_VALUE  # native fp64 needs no scratch
