@@ -247,6 +247,42 @@ ssblasStatus_t ssblasGemmGroupedBatchedEx(
     ssblasGemmAlgo_t         algo);
 
 /*
+ * Error report of the Ozaki fp64 form (no counterpart in the reference).
+ *
+ * Every call with SSBLAS_COMPUTE_64F_EMULATED_OZAKI[_S<n>] computes, per
+ * instance, an a-priori bound on its normwise error
+ *     ||C - C_exact||_F / (|alpha| ||op(A)||_F ||op(B)||_F)
+ * from the slice digits and the row / column sums of the operands (the
+ * truncation of the slices, the product triangle and the fp64 folds; the
+ * final rounding to fp64 that every fp64 result carries comes on top).  With
+ * the automatic compute type (..._OZAKI) each instance takes the smallest
+ * slice count, 3 to 9, whose bound is at most `target` (2^-55, a quarter of
+ * fp64's unit roundoff); an instance that misses it at 9 slices keeps 9 and
+ * is counted in over_target.  Rows or columns holding inf / nan give a bound
+ * of +inf (their outputs are NaN).
+ *
+ * ssblasGetOzakiReport waits for `stream` and returns the report of the last
+ * Ozaki call enqueued on it from this device (NOT_COMPUTED_ERROR if none):
+ * the summary, and per instance the bound and the slice count used (the
+ * first `capacity` instances into bounds[] / slices[], either may be NULL).
+ */
+typedef struct {
+    SSINT  instances;     /* batchCount of the call */
+    int    slices_min;    /* fewest / most slices any instance used */
+    int    slices_max;
+    double bound_max;     /* worst instance's bound */
+    double target;        /* 2^-55 */
+    SSINT  over_target;   /* instances whose bound exceeds target */
+} ssblasOzakiReport_t;
+
+ssblasStatus_t ssblasGetOzakiReport(
+    ssblasStream_t        stream,
+    ssblasOzakiReport_t*  report,
+    float*                bounds,
+    signed char*          slices,
+    SSINT                 capacity);
+
+/*
  * Which kernel a call with these arguments would run (no counterpart in the
  * reference; for tests and tuning): writes the tile-config name (e.g.
  * "dma256k2", "tile256x128", "direct16") or the Ozaki kernel family

@@ -171,6 +171,29 @@ def ssblasGemmBatchedExNoAlloc(transa, transb, m, n, k, alpha, A, Atype, lda, B,
     return Status(_lib.load().ssblasGemmBatchedExNoAlloc(*args, s))
 
 
+class OzakiReport(ctypes.Structure):
+    """ssblasOzakiReport_t of include/ssblasBatch.h."""
+    _fields_ = [("instances", ctypes.c_long), ("slices_min", ctypes.c_int), ("slices_max", ctypes.c_int),
+                ("bound_max", ctypes.c_double), ("target", ctypes.c_double), ("over_target", ctypes.c_long)]
+
+
+def ozaki_report(stream=0, per_instance=True):
+    """(Status, OzakiReport, bounds, slices) of the last Ozaki call on
+    `stream`; bounds / slices are numpy arrays (per instance) or None."""
+    import numpy as np
+
+    rep = OzakiReport()
+    s = stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream or 0)
+    lib = _lib.load()
+    st = Status(lib.ssblasGetOzakiReport(s, ctypes.byref(rep), None, None, 0))
+    if st != Status.SUCCESS or not per_instance:
+        return st, rep, None, None
+    b = np.empty(rep.instances, dtype=np.float32)
+    sl = np.empty(rep.instances, dtype=np.int8)
+    st = Status(lib.ssblasGetOzakiReport(s, ctypes.byref(rep), b.ctypes.data, sl.ctypes.data, rep.instances))
+    return st, rep, b, sl
+
+
 NOALLOC_SCRATCH_BYTES = 64 << 20  # SSBLAS_NOALLOC_SCRATCH_BYTES of include/ssblasBatch.h
 
 
@@ -215,5 +238,5 @@ from .grouped import GroupedPlan, ssblasGemmGroupedBatchedEx  # noqa: E402,F401
 __all__ = [
     "Status", "Operation", "DataType", "ComputeType", "GemmAlgo", "scale_ctype",
     "ssblasGemmBatchedEx", "ssblasGemmBatchedEx_64", "ssblasGemmBatchedExAsync", "ssblasGemmBatchedExNoAlloc",
-    "ssblasGemmGroupedBatchedEx", "GroupedPlan", "pointer_array", "pointer_array_strided", "kernel_config", "version",
+    "ssblasGemmGroupedBatchedEx", "GroupedPlan", "pointer_array", "pointer_array_strided", "kernel_config", "OzakiReport", "ozaki_report", "version",
 ]

@@ -48,6 +48,7 @@ SIGNATURES = {
     "ssblasGemmGroupedBatchedEx": (_c_int, _GROUPED_ARGS),
     "ssblasGetKernelConfig": (_c_int, [_c_int, _c_int, _c_long, _c_long, _c_long, _c_int, _c_int, _c_int, _c_long,
                                        _c_long, _c_long, _c_long, _c_int, ctypes.c_char_p, _c_long]),
+    "ssblasGetOzakiReport": (_c_int, [_vp, _vp, _vp, _vp, _c_long]),
     "ssblasGetVersion": (_c_int, []),
 }
 

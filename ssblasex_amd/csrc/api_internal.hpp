@@ -27,6 +27,22 @@ int pick_config(const GemmProblem& g);
 // Autotuned config for the problem's size bucket, -1 when none.
 int tuned_config(const GemmProblem& g);
 
+// Workspace of the Ozaki form: the per-instance report (error bound, slice
+// count) at the front -- all the fused form (m, n, k <= 32) needs -- then,
+// for the general path, the int8 slices of op(A) and op(B) (each kept in its
+// operand's storage orientation, leading dimension rounded up to 16 bytes)
+// and the int32 exponent vectors.
+struct OzakiLayout {
+    long ld_sa, cols_sa, ld_sb, cols_sb;  // slice matrices: storage ld and column count
+    size_t off_bound, off_s;              // float bound[batch], int8 slices[batch]
+    size_t off_a, off_b, off_e, total;    // A's and B's slices, the exponents; total bytes
+};
+// Where an Ozaki call left its per-instance report (device pointers).
+struct OzakiReportLoc {
+    const float* bound = nullptr;
+    const int8_t* slices = nullptr;
+    long batch = 0;
+};
 // A device workspace provider: returns a buffer of at least the requested
 // bytes (or nullptr with *err set).
 using WsGet = void* (*)(size_t bytes, hipError_t* err);
@@ -56,24 +72,19 @@ ssblasStatus_t run_prepared(const PreparedCall& pc, hipStream_t stream, WsGet ws
 hipError_t launch_ozaki(const GemmProblem& g, int slices, hipStream_t stream, WsGet ws_get, ssblasStatus_t* st);
 // Device side of the Ozaki form (ozaki.hip); ws_get returns a device buffer
 // of at least the requested size owned by the caller's stream.
-hipError_t ozaki_run(const GemmProblem& g, int s, WsGet ws_get, hipStream_t st);
-// Workspace of the general Ozaki path: the int8 slices of op(A) and op(B)
-// (each kept in its operand's storage orientation, leading dimension rounded
-// up to 16 bytes), then the int32 exponent vectors.
-struct OzakiLayout {
-    long ld_sa, cols_sa, ld_sb, cols_sb;  // slice matrices: storage ld and column count
-    size_t off_b, off_e, total;           // byte offsets of B's slices and the exponents; total bytes
-};
+// slices == 0: the automatic slice count.  *rep receives where the
+// per-instance report lands.
+hipError_t ozaki_run(const GemmProblem& g, int slices, WsGet ws_get, hipStream_t st, OzakiReportLoc* rep);
 OzakiLayout ozaki_layout(const GemmProblem& g, int s);
 // Device workspace an Ozaki call of this shape needs with s slices (0 = the
-// automatic choice), in bytes (0 for the fused form).
+// automatic choice), in bytes.
 size_t ozaki_workspace_bytes(const GemmProblem& g, int s);
 // The fused one-wave Ozaki form applies (m, n, k <= 32).
 bool ozaki_fused_shape(const GemmProblem& g);
 // Kernel family an Ozaki call of this shape runs ("ozaki_small" for the
 // fused one-wave form, "ozaki_gemm" otherwise).
 const char* ozaki_kernel_name(const GemmProblem& g);
-// Slice count used when the compute type does not name one.
-constexpr int OZAKI_DEFAULT_SLICES = 9;
+// Most slices the automatic choice may take.
+constexpr int OZAKI_MAX_SLICES = 9;
 
 }  // namespace ssb

@@ -49,6 +49,13 @@ struct OzParams {
     int beta0;
     int tiles_m, tiles_n;
     long total;
+    // error report and slice choice (see oz_bound): per instance the bound
+    // (nullptr: not reported) and the slice count used; auto_s: choose the
+    // slice count per instance (p.s is then the most allowed)
+    float* bound;
+    int8_t* sarr;
+    float target;
+    int auto_s;
 };
 
 constexpr int OZ_NONFINITE = INT32_MIN;
@@ -125,11 +132,14 @@ __global__ void __launch_bounds__(256) ozaki_exp_rows(const void* const* X, long
 // slices: elementwise over the storage of one operand
 // ---------------------------------------------------------------------------
 // exp_by_col: the exponent of element (r, c) is e[c] (else e[r]).
+// s: the most slices (the workspace's plane count); sarr[inst] (when given)
+// the instance's own count: planes past it are not written (nor read).
 __global__ void __launch_bounds__(256) ozaki_slice(const void* const* X, long ld, int nrows, int ncols,
-                                                   const int32_t* e, int exp_by_col, int s, int8_t* w,
-                                                   long ldw)
+                                                   const int32_t* e, int exp_by_col, int s, const int8_t* sarr,
+                                                   int8_t* w, long ldw)
 {
     const int inst = blockIdx.z;
+    const int si = sarr != nullptr ? sarr[inst] : s;
     const int c = blockIdx.y;
     const int r0 = (blockIdx.x * 256 + threadIdx.x) * 4;
     if (r0 >= ldw) return;
@@ -147,7 +157,7 @@ __global__ void __launch_bounds__(256) ozaki_slice(const void* const* X, long ld
             v[q] = (ex == OZ_NONFINITE) ? 0.0 : ldexp(*gbl(x + r), -ex);
         }
     }
-    for (int p = 0; p < s; ++p) {
+    for (int p = 0; p < si; ++p) {
         uint32_t word = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -157,6 +167,219 @@ __global__ void __launch_bounds__(256) ozaki_slice(const void* const* X, long ld
             word |= ((uint32_t)(int32_t)t & 0xffu) << (8 * q);
         }
         *gbl(reinterpret_cast<uint32_t*>(out + p * plane)) = word;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// per-instance error bound and slice choice of the general path (the bound
+// of oz_bound below, over rows and columns of any count): one workgroup per
+// instance, one pass over op(A) and op(B) with the exponents of the exponent
+// kernels.  Row sums are kept relative to the largest exponent seen so far
+// (rescaled by 4^(old - new) when a larger one comes), so one pass suffices.
+// ---------------------------------------------------------------------------
+constexpr int OZ_SMIN = 3, OZ_NCAND = 9 - OZ_SMIN + 1;
+
+struct OzRowSums {
+    int emax = INT32_MIN;          // largest exponent of a nonzero row so far
+    float W = 0.f, L2 = 0.f;       // sum w, sum w a2
+    float A1 = 0.f, A1sq = 0.f;    // sum w a1, sum w a1^2
+    float X1[OZ_NCAND] = {}, X2[OZ_NCAND] = {};  // per candidate s: sum w x, sum w x^2
+    __device__ void rescale(int e)
+    {
+        if (e <= emax) return;
+        const float f = emax == INT32_MIN ? 0.f : __builtin_ldexpf(1.f, 2 * max(emax - e, -63));
+        W *= f;
+        L2 *= f;
+        A1 *= f;
+        A1sq *= f;
+#pragma unroll
+        for (int c = 0; c < OZ_NCAND; ++c) {
+            X1[c] *= f;
+            X2[c] *= f;
+        }
+        emax = e;
+    }
+};
+struct OzColSums {
+    int emax = INT32_MIN;
+    float W = 0.f, L2 = 0.f, B1 = 0.f, B1sq = 0.f;
+    __device__ void rescale(int e)
+    {
+        if (e <= emax) return;
+        const float f = emax == INT32_MIN ? 0.f : __builtin_ldexpf(1.f, 2 * max(emax - e, -63));
+        W *= f;
+        L2 *= f;
+        B1 *= f;
+        B1sq *= f;
+        emax = e;
+    }
+};
+
+__device__ __forceinline__ float oz_cand_c(int s) { return __builtin_ldexpf(1.f / 127.f, 7 * s - 53); }
+
+// block-wide (256 threads) sum and max through LDS
+__device__ __forceinline__ float oz_block_sum(float v, float* red)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+__device__ __forceinline__ int oz_block_max(int v, int* red)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return max(max(red[0], red[1]), max(red[2], red[3]));
+}
+
+__global__ void __launch_bounds__(256) ozaki_stats(const OzParams p)
+{
+    __shared__ float redf[4];
+    __shared__ int redi[4];
+    const int inst = blockIdx.x, tid = threadIdx.x;
+    const double* A = static_cast<const double*>(p.A[inst]);
+    const double* B = static_cast<const double*>(p.B[inst]);
+    const int32_t* ea = p.ea + (long)inst * p.m;
+    const int32_t* eb = p.eb + (long)inst * p.n;
+    const int smax = p.s;
+    bool bad = false;
+    int Da = 0, Db = 0;  // most digits any value of op(A) / op(B) needs (10: more than nine)
+    OzRowSums R;
+    for (int i = tid; i < p.m; i += 256) {
+        const int e = ea[i];
+        if (e == OZ_NONFINITE) {
+            bad = true;
+            continue;
+        }
+        float a1 = 0.f, a2 = 0.f, d[9] = {};
+        for (int kk = 0; kk < p.k; ++kk) {
+            const double x = *gbl(A + (p.ta ? (long)kk + (long)i * p.lda : (long)i + (long)kk * p.lda));
+            double v = ldexp(fabs(x), -e);
+            const float vf = (float)v;
+            a1 += vf;
+            a2 = __builtin_fmaf(vf, vf, a2);
+            int depth = 0;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {  // the slice kernel's digits, magnitudes only
+                v *= 128.0;
+                const double t = trunc(v);
+                v -= t;
+                d[q] += (float)t;
+                if (t != 0.0) depth = q + 1;
+            }
+            Da = max(Da, v != 0.0 ? 10 : depth);
+        }
+        if (a1 == 0.f) continue;  // all-zero row: its products are exact
+        R.rescale(e);
+        const float w = __builtin_ldexpf(1.f, 2 * max(e - R.emax, -63));
+        R.W += w;
+        R.L2 += w * a2;
+        R.A1 += w * a1;
+        R.A1sq += w * a1 * a1;
+        float S = 0.f;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            S += d[q];
+            if (q + 1 >= OZ_SMIN) {
+                const float x = S * (1.f / 128.f) + oz_cand_c(q + 1) * a1;
+                R.X1[q + 1 - OZ_SMIN] += w * x;
+                R.X2[q + 1 - OZ_SMIN] += w * x * x;
+            }
+        }
+    }
+    OzColSums Cs;
+    for (int j = tid; j < p.n; j += 256) {
+        const int e = eb[j];
+        if (e == OZ_NONFINITE) {
+            bad = true;
+            continue;
+        }
+        float b1 = 0.f, b2 = 0.f;
+        for (int kk = 0; kk < p.k; ++kk) {
+            const double x = *gbl(B + (p.tb ? (long)j + (long)kk * p.ldb : (long)kk + (long)j * p.ldb));
+            double v = ldexp(fabs(x), -e);
+            const float vf = (float)v;
+            b1 += vf;
+            b2 = __builtin_fmaf(vf, vf, b2);
+            int depth = 0;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                v *= 128.0;
+                const double t = trunc(v);
+                v -= t;
+                if (t != 0.0) depth = q + 1;
+            }
+            Db = max(Db, v != 0.0 ? 10 : depth);
+        }
+        if (b1 == 0.f) continue;
+        Cs.rescale(e);
+        const float w = __builtin_ldexpf(1.f, 2 * max(e - Cs.emax, -63));
+        Cs.W += w;
+        Cs.L2 += w * b2;
+        Cs.B1 += w * b1;
+        Cs.B1sq += w * b1 * b1;
+    }
+    bad = oz_block_max(bad ? 1 : 0, redi) != 0;
+    Da = oz_block_max(Da, redi);
+    Db = oz_block_max(Db, redi);
+    const int emax = oz_block_max(R.emax, redi), fmax = oz_block_max(Cs.emax, redi);
+    float bound = 0.f;
+    int s = smax;
+    if (bad) {
+        bound = __builtin_huge_valf();
+    } else if (emax == INT32_MIN || fmax == INT32_MIN) {
+        s = p.auto_s ? OZ_SMIN : smax;  // op(A) or op(B) all zero: exact zero product
+    } else {
+        R.rescale(emax);
+        Cs.rescale(fmax);
+        const float Wa = oz_block_sum(R.W, redf), A2 = oz_block_sum(R.L2, redf);
+        const float Wb = oz_block_sum(Cs.W, redf), B2 = oz_block_sum(Cs.L2, redf);
+        const float B1 = oz_block_sum(Cs.B1, redf), B1sq = oz_block_sum(Cs.B1sq, redf);
+        float X1[OZ_NCAND], X2[OZ_NCAND];
+#pragma unroll
+        for (int c = 0; c < OZ_NCAND; ++c) {
+            X1[c] = oz_block_sum(R.X1[c], redf);
+            X2[c] = oz_block_sum(R.X2[c], redf);
+        }
+        const float inv = 1.f / (A2 * B2);
+        const float A1 = oz_block_sum(R.A1, redf);
+        const float A1sq = oz_block_sum(R.A1sq, redf);
+        auto bound_at = [&](int sc) {
+            const float c = oz_cand_c(sc);
+            const int ci = sc - OZ_SMIN;
+            float q;
+            if (Da + Db - 1 <= sc) {  // exactly representable: the fold term only
+                q = c * c * (A1sq * Wb + 2.f * A1 * B1 + Wa * B1sq);
+            } else {
+                q = X2[ci] * Wb + 2.f * X1[ci] * (1.f + c) * B1 + Wa * (1.f + c) * (1.f + c) * B1sq;
+            }
+            return __builtin_ldexpf(__builtin_sqrtf(q * inv), -7 * sc);
+        };
+        // p.s is the fixed count or the most allowed (at least OZ_SMIN)
+        float b = 0.f;
+#pragma unroll
+        for (int sc = OZ_SMIN; sc <= 9; ++sc)
+            if (sc == smax) b = bound_at(sc);
+        bound = b;
+        if (p.auto_s && b <= p.target) {
+#pragma unroll
+            for (int sc = 8; sc >= OZ_SMIN; --sc) {
+                if (sc >= smax) continue;
+                const float bs = bound_at(sc);
+                if (!(bs <= p.target)) break;
+                s = sc;
+                bound = bs;
+            }
+        }
+    }
+    if (tid == 0) {
+        p.sarr[inst] = (int8_t)s;
+        if (p.bound != nullptr) p.bound[inst] = bound;
     }
 }
 
@@ -196,6 +419,7 @@ __global__ void __launch_bounds__(256) ozaki_gemm(const OzParams p)
         const int rows_a = min(BM, p.m - i0), rows_b = min(BN, p.n - j0);
         const char* Ai = reinterpret_cast<const char*>(p.wa) + (long)inst * s * plane_a + (long)i0 * sra;
         const char* Bi = reinterpret_cast<const char*>(p.wb) + (long)inst * s * plane_b + (long)j0 * srb;
+        const int si = p.sarr != nullptr ? p.sarr[inst] : s;  // this instance's slice count
         const bool a_vec = (p.ld_sa % SA::word_bytes(AKC)) == 0, b_vec = (p.ld_sb % SB::word_bytes(BKC)) == 0;
         const bool a_fast = a_vec && rows_a == BM, b_fast = b_vec && rows_b == BN;
 
@@ -208,8 +432,8 @@ __global__ void __launch_bounds__(256) ozaki_gemm(const OzParams p)
         for (int kc0 = 0; kc0 < p.k; kc0 += OZ_KCHUNK) {
             const int kc = min(OZ_KCHUNK, p.k - kc0);
             const int nk = (kc + BK - 1) / BK;
-            for (int tt = s + 1; tt >= 2; --tt) {
-                const int plo = max(1, tt - s), phi = min(s, tt - 1);
+            for (int tt = si + 1; tt >= 2; --tt) {
+                const int plo = max(1, tt - si), phi = min(si, tt - 1);
                 const int nsteps = (phi - plo + 1) * nk;
                 i32x4 acc[FM][FN];
 #pragma unroll
@@ -343,29 +567,51 @@ __device__ __forceinline__ uint32_t oz_spread7(uint32_t h)
 // digit 9 (every step exact in fp64); the 7-bit fields are spread to bytes
 // and transposed across 4 values; a negative value's bytes are negated at
 // once (bytes below 128: -d = ((0x80 - d) ^ 0x80) without borrows).
-template <int S>
-__device__ __forceinline__ void oz_slice_regs(const double* x, int e, u32x4 (&sl)[S])
-{
-    static_assert(S >= 1 && S <= 9, "1 to 9 slices");
-    uint32_t H[16], L[16], N[16], neg[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const double a = (e == OZ_NONFINITE) ? 0.0 : ldexp(fabs(x[i]), -e);
-        neg[i >> 2] |= (x[i] < 0.0 ? 0xffu : 0u) << (8 * (i & 3));
-        const double y1 = ldexp(a, 28);
-        H[i] = (uint32_t)y1;
-        if constexpr (S > 4) {
-            const double y2 = ldexp(y1 - (double)H[i], 28);
-            L[i] = (uint32_t)y2;
-            if constexpr (S > 8) N[i] = (uint32_t)ldexp(y2 - (double)L[i], 7);
-        }
+//
+// Statistics for the error bound (oz_bound): l1 += sum of the 16 scaled
+// magnitudes |v|, l2 += sum of v^2, dsum[p] += sum of the 16 magnitudes of
+// digit p (v_sad_u8 against zero adds the four bytes of a word).
+// Taken from the digit words, so that gathering them costs a few byte-dot
+// and byte-sum instructions per word: with d_p the p-th digit of a value,
+//   l1 = sum (d1 2^-7 + d2 2^-14 + d3 2^-21) + 16 * 2^-21  >= sum |v|
+//   l2 = sum (d1 2^-7 + d2 2^-14)^2                      <= sum v^2
+// over the lane's 16 values (truncated digits never exceed the value), so the
+// bound built on them stays an upper bound.
+struct OzStats {
+    uint32_t d1 = 0, d2 = 0, d3 = 0;     // digit sums
+    uint32_t u11 = 0, u12 = 0, u22 = 0;  // sums of d1 d1, d1 d2, d2 d2
+    __device__ float l1() const
+    {
+        return d1 == 0u ? 0.f : (float)d1 * 0x1p-7f + (float)d2 * 0x1p-14f + (float)d3 * 0x1p-21f + 16.f * 0x1p-21f;
     }
+    __device__ float l2() const { return ((float)u11 * 16384.f + (float)u12 * 256.f + (float)u22) * 0x1p-28f; }
+};
+template <int S, bool STATS = false, bool DIGITS = false>
+__device__ __forceinline__ void oz_slice_regs(const double* x, int e, u32x4 (&sl)[S], OzStats* st = nullptr,
+                                              uint32_t* dsum = nullptr)
+{
+    static_assert(S >= 3 && S <= 9, "3 to 9 slices");
+    // four values (one 32-bit word of each slice) at a time, so that only
+    // their digit fields are live
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
+        uint32_t H[4], L[4], N[4], neg = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 4 * j + q;
+            const double a = (e == OZ_NONFINITE) ? 0.0 : ldexp(fabs(x[i]), -e);
+            neg |= (x[i] < 0.0 ? 0xffu : 0u) << (8 * q);
+            const double y1 = ldexp(a, 28);
+            H[q] = (uint32_t)y1;
+            if constexpr (S > 4) {
+                const double y2 = ldexp(y1 - (double)H[q], 28);
+                L[q] = (uint32_t)y2;
+                if constexpr (S > 8) N[q] = (uint32_t)ldexp(y2 - (double)L[q], 7);
+            }
+        }
         uint32_t w[9];
         {
-            const uint32_t in[4] = {oz_spread7(H[4 * j]), oz_spread7(H[4 * j + 1]), oz_spread7(H[4 * j + 2]),
-                                    oz_spread7(H[4 * j + 3])};
+            const uint32_t in[4] = {oz_spread7(H[0]), oz_spread7(H[1]), oz_spread7(H[2]), oz_spread7(H[3])};
             uint32_t out[4];
             transpose4x4_bytes(in, out);  // out[c]: byte c of the four values
             w[0] = out[3];
@@ -374,8 +620,7 @@ __device__ __forceinline__ void oz_slice_regs(const double* x, int e, u32x4 (&sl
             w[3] = out[0];
         }
         if constexpr (S > 4) {
-            const uint32_t in[4] = {oz_spread7(L[4 * j]), oz_spread7(L[4 * j + 1]), oz_spread7(L[4 * j + 2]),
-                                    oz_spread7(L[4 * j + 3])};
+            const uint32_t in[4] = {oz_spread7(L[0]), oz_spread7(L[1]), oz_spread7(L[2]), oz_spread7(L[3])};
             uint32_t out[4];
             transpose4x4_bytes(in, out);
             w[4] = out[3];
@@ -383,14 +628,114 @@ __device__ __forceinline__ void oz_slice_regs(const double* x, int e, u32x4 (&sl
             w[6] = out[1];
             w[7] = out[0];
         }
-        if constexpr (S > 8) w[8] = N[4 * j] | (N[4 * j + 1] << 8) | (N[4 * j + 2] << 16) | (N[4 * j + 3] << 24);
+        if constexpr (S > 8) w[8] = N[0] | (N[1] << 8) | (N[2] << 16) | (N[3] << 24);
 #pragma unroll
         for (int p = 0; p < S; ++p) {
             const uint32_t nw = (0x80808080u - w[p]) ^ 0x80808080u;
-            sl[p][j] = (neg[j] & nw) | (~neg[j] & w[p]);
+            sl[p][j] = (neg & nw) | (~neg & w[p]);
+            if constexpr (DIGITS) dsum[p] = __builtin_amdgcn_sad_u8(w[p], 0u, dsum[p]);
+        }
+        if constexpr (STATS) {
+            if constexpr (DIGITS) {
+                st->d1 = dsum[0];
+                st->d2 = dsum[1];
+                st->d3 = dsum[2];
+            } else {
+                st->d1 = __builtin_amdgcn_sad_u8(w[0], 0u, st->d1);
+                st->d2 = __builtin_amdgcn_sad_u8(w[1], 0u, st->d2);
+                st->d3 = __builtin_amdgcn_sad_u8(w[2], 0u, st->d3);
+            }
+            st->u11 = __builtin_amdgcn_udot4(w[0], w[0], st->u11, false);
+            st->u12 = __builtin_amdgcn_udot4(w[0], w[1], st->u12, false);
+            st->u22 = __builtin_amdgcn_udot4(w[1], w[1], st->u22, false);
         }
     }
 }
+
+// ---------------------------------------------------------------------------
+// a-priori error bound of the Ozaki product, per instance
+//
+// With v = x * 2^-e (|v| < 1) cut into digits v = sum_{p<=s} v_p 2^-7p + r,
+// |r| < 2^-7s, the triangle p + q <= s + 1 computes, for one product a*b,
+// sum_p a_p 2^-7p * b_(<= s+1-p)  (b cut after s+1-p digits), so
+//     |a*b - computed| <= 2^-7(s+1) sum_p |a_p| + 2^-7s |b|
+// (b's remainder after j digits is below 2^-7j; a's after s below 2^-7s).
+// The fp64 folds of the orders t = s+1 .. 3 add partial sums below
+// 2^-7(t-2) (|a| + |b|) summed over k, so their roundings stay below
+// u/127 (a1_i + b1_j); the last fold (t = 2) is the final rounding every
+// fp64 result carries and is not part of the bound.  Per element of
+// op(A) op(B) (row i, column j; a1_i = sum_k |a'_ik|, b1_j likewise,
+// S_i(s) = sum_k sum_{p<=s} |a_p|):
+//     |dP_ij| <= 2^(e_i+f_j) 2^-7s (x_i + y_j),
+//     x_i = S_i(s)/128 + c_s a1_i,  y_j = (1 + c_s) b1_j,  c_s = 2^(7s-53)/127.
+// Summed over (i, j) the square separates into row and column sums:
+//     ||dP||_F^2 <= 2^-14s (X2 Wb + 2 X1 (1+c_s) B1 + Wa (1+c_s)^2 B1sq) * 4^(emax+fmax)
+// with weights w_i = 4^(e_i-emax) (0 for an all-zero row), Wa = sum w_i,
+// X1 = sum w_i x_i, X2 = sum w_i x_i^2, and wb_j, Wb, B1 = sum wb_j b1_j,
+// B1sq = sum wb_j b1_j^2 for the columns; ||op(A)||_F^2 = 4^emax sum w_i a2_i
+// (a2_i = sum_k a'_ik^2), likewise op(B).  The normwise bound
+//     ||dP||_F / (||op(A)||_F ||op(B)||_F)
+// is evaluated in fp32 (an estimate of a bound: 2^-24 relative is ample).
+// Exactly representable operands: when every value of op(A) needs at most
+// Da digits and every value of op(B) at most Db, and Da + Db - 1 <= s, no
+// digit is cut and no product of the triangle is dropped, so only the fold
+// term remains (x_i = c_s a1_i, y_j = c_s b1_j).
+// Auto mode takes the smallest s (<= p.s) whose bound is at most the
+// target; an instance that misses it at p.s keeps p.s and reports its bound.
+// Rows or columns with inf / nan: bound +inf (their outputs are NaN).
+// ---------------------------------------------------------------------------
+constexpr float OZ_TARGET = 2.7755575615628914e-17f;  // 2^-55, a quarter unit roundoff of fp64
+
+// Sum over the 32 row (column) owners; lanes l and l ^ 32 hold the same values.
+__device__ __forceinline__ float oz_sum32(float v)
+{
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ __forceinline__ int oz_max32(int v)
+{
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    return v;
+}
+
+// Bound at S slices for the fused one-wave form: lane l owns row l % 32 of
+// op(A) and column l % 32 of op(B) over the k half l / 32 (the statistics of
+// both halves are joined first).  The fused kernel's time is set by its
+// loads and its slicing, not by the slice count, so its automatic choice is
+// always the most slices and only the bound at S is evaluated here; the
+// general path chooses per instance (ozaki_stats).
+template <int S>
+__device__ __forceinline__ float oz_bound(int ea, int eb, OzStats sa, OzStats sb, const uint32_t (&dsum)[S])
+{
+    if (__any(ea == OZ_NONFINITE || eb == OZ_NONFINITE)) return __builtin_huge_valf();
+    // join the two k halves
+    float a1 = sa.l1(), a2 = sa.l2(), b1 = sb.l1(), b2 = sb.l2();
+    a1 += __shfl_xor(a1, 32);
+    a2 += __shfl_xor(a2, 32);
+    b1 += __shfl_xor(b1, 32);
+    b2 += __shfl_xor(b2, 32);
+    uint32_t Sa = 0;  // digit sum S_i(S)
+#pragma unroll
+    for (int p = 0; p < S; ++p) Sa += dsum[p];
+    Sa += (uint32_t)__shfl_xor((int)Sa, 32);
+    const bool rowz = a1 == 0.f, colz = b1 == 0.f;
+    const int emax = oz_max32(rowz ? INT32_MIN : ea), fmax = oz_max32(colz ? INT32_MIN : eb);
+    if (emax == INT32_MIN || fmax == INT32_MIN) return 0.f;  // op(A) or op(B) all zero: exact zero product
+    const float wa = rowz ? 0.f : __builtin_ldexpf(1.f, 2 * max(ea - emax, -63));
+    const float wb = colz ? 0.f : __builtin_ldexpf(1.f, 2 * max(eb - fmax, -63));
+    const float Wa = oz_sum32(wa), A2 = oz_sum32(wa * a2);
+    const float Wb = oz_sum32(wb), B2 = oz_sum32(wb * b2);
+    const float B1 = oz_sum32(wb * b1), B1sq = oz_sum32(wb * b1 * b1);
+    const float c = __builtin_ldexpf(1.f / 127.f, 7 * S - 53);
+    const float x = (float)Sa * (1.f / 128.f) + c * a1;
+    const float cy = 1.f + c;
+    const float X1 = oz_sum32(wa * x), X2 = oz_sum32(wa * x * x);
+    const float q = X2 * Wb + 2.f * X1 * cy * B1 + Wa * cy * cy * B1sq;
+    return __builtin_ldexpf(__builtin_sqrtf(q / (A2 * B2)), -7 * S);
+}
+
 
 // Exponent of the max |.| over both k halves (lanes l and l ^ 32).
 __device__ __forceinline__ int oz_exp16(const double* x)
@@ -441,18 +786,27 @@ __device__ __forceinline__ T* wave_uniform(T* ptr)
 }
 
 // FULL: the instance is a whole 32 x 32 x 32 block (no element to zero).
+// The lane's offsets go through an empty asm per instance: left visible, the
+// compiler hoists all 16 of them out of the instance loop and keeps them
+// live in registers across it.
+__device__ __forceinline__ uint32_t oz_opaque(uint32_t x)
+{
+    __asm__ volatile("" : "+v"(x));
+    return x;
+}
 template <bool FULL>
 __device__ __forceinline__ void oz_small_load(const void* X, const OzLane& L, double (&x)[16])
 {
     const double* P = wave_uniform(static_cast<const double*>(X));
+    const uint32_t base = oz_opaque(L.base), step = oz_opaque(L.step);
     if constexpr (FULL) {
 #pragma unroll
-        for (int q = 0; q < 16; ++q) x[q] = *gbl(P + (L.base + (uint32_t)q * L.step));
+        for (int q = 0; q < 16; ++q) x[q] = *gbl(P + (base + (uint32_t)q * step));
     } else {
 #pragma unroll
         for (int q = 0; q < 16; ++q) {
             const uint32_t qq = (uint32_t)q < L.kvalid ? (uint32_t)q : 0u;  // stays inside the matrix
-            const double v = *gbl(P + (L.base + qq * L.step));
+            const double v = *gbl(P + (base + qq * step));
             x[q] = (L.row_ok && (uint32_t)q < L.kvalid) ? v : 0.0;
         }
     }
@@ -485,10 +839,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
         oz_small_load<FULL>(p.A[inst], LA, xa);
         oz_small_load<FULL>(p.B[inst], LB, xb);
         u32x4 sa[S], sb[S];
+        OzStats sta, stb;
+        uint32_t dsum[S];
+#pragma unroll
+        for (int q = 0; q < S; ++q) dsum[q] = 0u;
         const int ea = oz_exp16(xa);
-        oz_slice_regs<S>(xa, ea, sa);
+        oz_slice_regs<S, true, true>(xa, ea, sa, &sta, dsum);
         const int eb = oz_exp16(xb);
-        oz_slice_regs<S>(xb, eb, sb);
+        oz_slice_regs<S, true, false>(xb, eb, sb, &stb);
 
         double D[16];
 #pragma unroll
@@ -553,6 +911,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
                 }
             }
         }
+        // the instance's error bound, from the statistics gathered while
+        // slicing (lanes beyond the instance hold zeros); computed after the
+        // stores so that its reductions do not add to the product loop's
+        // registers
+        if (p.bound != nullptr) {
+            const float bound = oz_bound<S>(ea, eb, sta, stb, dsum);
+            if (lane == 0) {
+                p.bound[inst] = bound;
+                p.sarr[inst] = (int8_t)S;
+            }
+        }
     }
 }
 
@@ -609,56 +978,40 @@ static hipError_t run_ozaki_gemm(OzParams p, hipStream_t st)
     return hipGetLastError();
 }
 
-// Exponents + slices of one operand.  Storage is nrows x ncols (column
-// major); exponents are per storage column when exp_by_col.
-static hipError_t prep_operand(const void* const* X, long ld, int nrows, int ncols, int exp_by_col, int s,
-                               int batch, int32_t* e, int8_t* w, long ldw, hipStream_t st)
+// Exponents of one operand's rows (op(A)) or columns (op(B)).  Storage is
+// nrows x ncols (column major); exponents are per storage column when
+// exp_by_col.
+static hipError_t exp_operand(const void* const* X, long ld, int nrows, int ncols, int exp_by_col, int batch,
+                              int32_t* e, hipStream_t st)
 {
     if (exp_by_col) {
         hipLaunchKernelGGL(ozaki_exp_cols, dim3((ncols + 3) / 4, batch), dim3(256), 0, st, X, ld, nrows, ncols, e);
     } else {
         hipLaunchKernelGGL(ozaki_exp_rows, dim3((nrows + 63) / 64, batch), dim3(256), 0, st, X, ld, nrows, ncols, e);
     }
-    hipError_t err = hipGetLastError();
-    if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(ozaki_slice, dim3((unsigned)((ldw / 4 + 255) / 256), ncols, batch), dim3(256), 0, st, X, ld,
-                       nrows, ncols, e, exp_by_col, s, w, ldw);
     return hipGetLastError();
 }
 
-hipError_t ozaki_run(const GemmProblem& g, int s, WsGet ws_get, hipStream_t st)
+static hipError_t slice_operand(const void* const* X, long ld, int nrows, int ncols, int exp_by_col, int s,
+                                const int8_t* sarr, int batch, const int32_t* e, int8_t* w, long ldw,
+                                hipStream_t st)
+{
+    hipLaunchKernelGGL(ozaki_slice, dim3((unsigned)((ldw / 4 + 255) / 256), ncols, batch), dim3(256), 0, st, X, ld,
+                       nrows, ncols, e, exp_by_col, s, sarr, w, ldw);
+    return hipGetLastError();
+}
+
+hipError_t ozaki_run(const GemmProblem& g, int slices, WsGet ws_get, hipStream_t st, OzakiReportLoc* rep)
 {
     const int m = g.m, n = g.n, k = g.k;
-    if (ozaki_fused_shape(g) && s >= 3 && s <= 9) {
-        // fused single-wave form: no workspace
-        OzParams p{};
-        p.A = g.A;
-        p.B = g.B;
-        p.C = g.C;
-        p.lda = g.lda;
-        p.ldb = g.ldb;
-        p.ldc = g.ldc;
-        p.m = m;
-        p.n = n;
-        p.k = k;
-        p.batch = g.batch;
-        p.s = s;
-        p.ta = g.ta;
-        p.tb = g.tb;
-        p.sc.ad = g.ad;
-        p.sc.bd = g.bd;
-        p.beta0 = g.beta0 ? 1 : 0;
-        return ozaki_small_dispatch(p, st);
-    }
+    const bool auto_s = slices == 0;
+    const int s = auto_s ? 9 : slices;  // the most slices an instance may take
     const OzakiLayout L = ozaki_layout(g, s);
     hipError_t err = hipSuccess;
     char* ws = static_cast<char*>(ws_get(L.total, &err));
     if (ws == nullptr) return err == hipSuccess ? hipErrorOutOfMemory : err;
-    const long ra = g.ta ? k : m, ca = g.ta ? m : k;  // storage of A
-    const long rb = g.tb ? n : k, cb = g.tb ? k : n;  // storage of B
-    const long ld_sa = L.ld_sa, ld_sb = L.ld_sb;
 
-    OzParams p;
+    OzParams p{};
     p.A = g.A;
     p.B = g.B;
     p.C = g.C;
@@ -672,14 +1025,6 @@ hipError_t ozaki_run(const GemmProblem& g, int s, WsGet ws_get, hipStream_t st)
     p.s = s;
     p.ta = g.ta;
     p.tb = g.tb;
-    p.wa = reinterpret_cast<int8_t*>(ws);
-    p.wb = reinterpret_cast<int8_t*>(ws + L.off_b);
-    p.ea = reinterpret_cast<int32_t*>(ws + L.off_e);
-    p.eb = p.ea + (size_t)g.batch * m;
-    p.ld_sa = ld_sa;
-    p.cols_sa = ca;
-    p.ld_sb = ld_sb;
-    p.cols_sb = cb;
     p.sc.af = g.af;
     p.sc.bf = g.bf;
     p.sc.ad = g.ad;
@@ -687,12 +1032,40 @@ hipError_t ozaki_run(const GemmProblem& g, int s, WsGet ws_get, hipStream_t st)
     p.sc.ai = g.ai;
     p.sc.bi = g.bi;
     p.beta0 = g.beta0 ? 1 : 0;
+    p.bound = reinterpret_cast<float*>(ws + L.off_bound);
+    p.sarr = reinterpret_cast<int8_t*>(ws + L.off_s);
+    p.target = OZ_TARGET;
+    p.auto_s = auto_s ? 1 : 0;
+    rep->bound = p.bound;
+    rep->slices = p.sarr;
+    rep->batch = g.batch;
+
+    if (ozaki_fused_shape(g)) return ozaki_small_dispatch(p, st);  // slices in registers: no more workspace
+
+    const long ra = g.ta ? k : m, ca = g.ta ? m : k;  // storage of A
+    const long rb = g.tb ? n : k, cb = g.tb ? k : n;  // storage of B
+    p.wa = reinterpret_cast<int8_t*>(ws + L.off_a);
+    p.wb = reinterpret_cast<int8_t*>(ws + L.off_b);
+    p.ea = reinterpret_cast<int32_t*>(ws + L.off_e);
+    p.eb = p.ea + (size_t)g.batch * m;
+    p.ld_sa = L.ld_sa;
+    p.cols_sa = ca;
+    p.ld_sb = L.ld_sb;
+    p.cols_sb = cb;
 
     // op(A) row i: storage row i (N) -> reduce along the strided dim; storage column i (T)
-    err = prep_operand(g.A, g.lda, (int)ra, (int)ca, g.ta ? 1 : 0, s, g.batch, p.ea, p.wa, ld_sa, st);
+    err = exp_operand(g.A, g.lda, (int)ra, (int)ca, g.ta ? 1 : 0, g.batch, p.ea, st);
     if (err != hipSuccess) return err;
     // op(B) column j: storage column j (N); storage row j (T)
-    err = prep_operand(g.B, g.ldb, (int)rb, (int)cb, g.tb ? 0 : 1, s, g.batch, p.eb, p.wb, ld_sb, st);
+    err = exp_operand(g.B, g.ldb, (int)rb, (int)cb, g.tb ? 0 : 1, g.batch, p.eb, st);
+    if (err != hipSuccess) return err;
+    // error bound and slice count per instance
+    hipLaunchKernelGGL(ozaki_stats, dim3((unsigned)g.batch), dim3(256), 0, st, p);
+    err = hipGetLastError();
+    if (err != hipSuccess) return err;
+    err = slice_operand(g.A, g.lda, (int)ra, (int)ca, g.ta ? 1 : 0, s, p.sarr, g.batch, p.ea, p.wa, L.ld_sa, st);
+    if (err != hipSuccess) return err;
+    err = slice_operand(g.B, g.ldb, (int)rb, (int)cb, g.tb ? 0 : 1, s, p.sarr, g.batch, p.eb, p.wb, L.ld_sb, st);
     if (err != hipSuccess) return err;
 
     // slice operand views: A-T / B-N are K-contiguous

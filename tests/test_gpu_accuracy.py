@@ -6,7 +6,7 @@ error of each form can be reported beside its throughput.  Bars:
 * native fp32 / fp64 <= 2 x reference-order error + 1 unit roundoff
 * BF16X6 <= 4 x reference-order fp32 error + 1 unit roundoff
 * BF16X3 <= 2^-14
-* Ozaki, default slice count (9) <= reference-order fp64 error + 1 unit
+* Ozaki, automatic slice count <= reference-order fp64 error + 1 unit
   roundoff ("no worse than the plain path" with the same rounding slack)
 """
 from __future__ import annotations

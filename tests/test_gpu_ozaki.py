@@ -2,10 +2,13 @@
 
 The form has no counterpart in the reference; its parity bar is the
 north-star one: normwise error against the FP64 (double-double) oracle,
-reported per slice count, and at the default slice count (8) no worse than
+reported per slice count, and with the automatic slice count no worse than
 2 x the reference-order fp64 restatement's error + 1 unit roundoff.  In
 addition the library must reproduce tests/ozaki_model.py (the same
-algorithm in numpy) bit for bit when alpha is a power of two.
+algorithm in numpy) bit for bit when alpha is a power of two, at the slice
+count each instance used, and its per-instance error report
+(ssblasGetOzakiReport) must equal the model's a-priori bound and hold
+against the double-double oracle.
 """
 from __future__ import annotations
 
@@ -13,8 +16,8 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import UNIT_ROUNDOFF, make_batch, normwise, padding_untouched, run_gpu, run_oracle
-from ozaki_model import ozaki_gemm
+from helpers import UNIT_ROUNDOFF, dd_reference, make_batch, normwise, padding_untouched, run_gpu, run_oracle
+from ozaki_model import U, auto_slices, error_bound, error_bound_fused, ozaki_gemm
 
 pytestmark = pytest.mark.gpu
 OPS = [(0, 0), (0, 1), (1, 0), (1, 1)]
@@ -56,7 +59,7 @@ def test_ozaki_error_decreases_and_meets_native(ta, tb):
         assert errs[s + 1] < errs[s] or errs[s + 1] <= UNIT_ROUNDOFF["f64"], errs
     _, R = run_oracle(b)
     e_ref = normwise(b, R)
-    st, C = run_gpu(b, compute_type=6)  # default slice count (9)
+    st, C = run_gpu(b, compute_type=6)  # automatic slice count
     assert st == 0
     assert normwise(b, C) <= 2 * e_ref + UNIT_ROUNDOFF["f64"], (normwise(b, C), e_ref, errs)
 
@@ -150,3 +153,100 @@ def test_ozaki_fused_full_blocks(ta, tb, pad, monkeypatch):
     for i, (x, y) in enumerate(zip(fused, general)):
         assert np.array_equal(x, y), f"instance {i}"
 
+
+
+def report():
+    import torch
+
+    import ssblasex_amd as S
+
+    st, rep, bounds, slices = S.ozaki_report(torch.cuda.current_stream())
+    assert st == S.Status.SUCCESS, st
+    return rep, bounds, slices
+
+
+REPORT_CASES = [((32, 32, 32), 0x109), ((32, 32, 32), 6), ((20, 31, 17), 0x105), ((32, 32, 32), 0x107),
+                ((48, 40, 96), 0x109), ((48, 40, 96), 6), ((70, 33, 65), 0x104), ((64, 64, 64), 0x108)]
+
+
+@pytest.mark.parametrize("ta,tb", OPS)
+@pytest.mark.parametrize("shape,ct", REPORT_CASES, ids=[f"{c[0]}-{c[1]:#x}" for c in REPORT_CASES])
+def test_ozaki_report_bound_holds(ta, tb, shape, ct):
+    """Per instance: the reported slice count is the named one (or the
+    model's automatic choice), the reported bound equals the model's bound
+    (fp32 evaluation: 1e-3 relative), the result equals the model at that
+    slice count bit for bit, and the error against the double-double oracle
+    stays within the bound plus the final fp64 rounding (one unit roundoff
+    of the normwise scale).  Fused (m, n, k <= 32) and general paths."""
+    m, n, k = shape
+    b = make_batch("f64", ta, tb, m, n, k, 6, alpha=1.0, beta=0.0, pad=(1, 0, 2), seed=ct + m, dist="wide")
+    # rows and columns of very different scales, a zero row and a zero column
+    oracle.colmajor(b.A[1], *b.dims_a(), b.lda)[:] *= 2.0 ** 40
+    A2 = oracle.colmajor(b.A[2], *b.dims_a(), b.lda)
+    if b.transa == 0:
+        A2[0, :] = 0.0
+    else:
+        A2[:, 0] = 0.0
+    B3 = oracle.colmajor(b.B[3], *b.dims_b(), b.ldb)
+    if b.transb == 0:
+        B3[:, 0] = 0.0
+    else:
+        B3[0, :] = 0.0
+    st, C = run_gpu(b, compute_type=ct)
+    assert st == 0
+    rep, bounds, slices = report()
+    assert rep.instances == b.batch
+    fused = max(m, n, k) <= 32  # the fused one-wave kernel: its automatic choice is always 9 slices
+    for i in range(b.batch):
+        A, B = op_mats(b, i)
+        s = (9 if fused else auto_slices(A, B)) if ct == 6 else ct - 0x100
+        assert slices[i] == s, (i, slices[i], s)
+        want_bound = (error_bound_fused if fused else error_bound)(A, B, s)
+        assert abs(bounds[i] - want_bound) <= 1e-3 * want_bound + 1e-30, (i, bounds[i], want_bound)
+        got = oracle.colmajor(C[i], m, n, b.ldc)
+        assert np.array_equal(got, ozaki_gemm(A, B, s)), i
+        hi, lo = dd_reference(b, i)
+        err = oracle.normwise_error(got, hi, lo, np.linalg.norm(A) * np.linalg.norm(B))
+        assert err <= bounds[i] * (1 + 1e-3) + U, (i, s, err, bounds[i])
+    assert rep.bound_max == pytest.approx(float(bounds.max()))
+    assert rep.slices_min == slices.min() and rep.slices_max == slices.max()
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (50, 40, 60)], ids=str)
+def test_ozaki_auto_exact_data_takes_few_slices(shape):
+    """Integer-valued operands need one or two digits: the general path's
+    automatic choice takes 3 slices (the fused kernel always takes 9), the
+    general path's reported bound is the fold term only, and the result is
+    the exact integer product."""
+    m, n, k = shape
+    b = make_batch("f64", 0, 1, m, n, k, 4, alpha=1.0, beta=0.0, seed=21)
+    rng = np.random.default_rng(22)
+    for x in b.A + b.B:
+        x[:] = rng.integers(-100, 101, x.shape)
+    st, C = run_gpu(b, compute_type=6)
+    assert st == 0
+    rep, bounds, slices = report()
+    assert (slices == (9 if max(shape) <= 32 else 3)).all(), slices
+    if max(shape) > 32:
+        assert (bounds <= 2.0 ** -55).all(), bounds
+    for i in range(b.batch):
+        A, B = op_mats(b, i)
+        exact = (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float64)
+        assert np.array_equal(oracle.colmajor(C[i], m, n, b.ldc), exact)
+
+
+def test_ozaki_report_nonfinite_and_absent():
+    import torch
+
+    import ssblasex_amd as S
+
+    s2 = torch.cuda.Stream()
+    st, rep, _, _ = S.ozaki_report(s2)
+    assert st == S.Status.NOT_COMPUTED_ERROR
+    b = make_batch("f64", 0, 0, 20, 12, 30, 2, alpha=1.0, beta=0.0, seed=4)
+    oracle.colmajor(b.A[1], 20, 30, b.lda)[3, 7] = np.nan
+    st, C = run_gpu(b, compute_type=6)
+    assert st == 0
+    rep, bounds, slices = report()
+    assert np.isfinite(bounds[0]) and np.isinf(bounds[1])
+    assert rep.over_target == 1 and np.isinf(rep.bound_max)

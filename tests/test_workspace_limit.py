@@ -57,9 +57,13 @@ def test_plain_forms_need_no_workspace(limit):
     assert st == S.Status.INVALID_VALUE
 
 
-def test_fused_small_ozaki_needs_no_workspace(limit):
-    limit("0")
+def test_fused_small_ozaki_needs_only_its_report(limit):
+    # m, n, k <= 32: slices stay in registers; the workspace is the
+    # per-instance report (4-byte bound + 1-byte slice count, 256-byte aligned)
+    limit(str(65536 * 5 + 256))
     assert ozaki_call(m=32, batch=65536) == S.Status.INVALID_VALUE
+    limit(str(65536 * 5 - 1))
+    assert ozaki_call(m=32, batch=65536) == S.Status.ALLOC_FAILED
 
 
 def test_grouped_plan_over_limit(limit):
