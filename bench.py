@@ -443,6 +443,15 @@ def run_config(cid, args, stream, forms=None, cpu=True, threads=1):
         if err is not None:
             ent["normwise_error"] = err
             ent["ratio_to_reference_order"] = round(err / check.ref_err, 4) if check.ref_err > 0 else None
+        if fname.startswith("ozaki"):
+            # the library's own a-priori bound for the last (timed) call, every instance
+            import ssblasex_amd as S
+
+            st, rep, _, _ = S.ozaki_report(stream, per_instance=False)
+            if st == S.Status.SUCCESS:
+                ent["reported_bound_max"] = rep.bound_max
+                ent["reported_slices"] = [rep.slices_min, rep.slices_max]
+                ent["instances_over_target"] = rep.over_target
         res["forms"][fname] = ent
     if cpu and cpu_inst:
         res["cpu_baseline"] = cpu_baseline_blas(prob, threads, cpu_inst)
