@@ -209,11 +209,21 @@ class Problem:
             self.Cflat.copy_(saved)
 
     def call(self, compute_type=0, stream=None):
+        """One ssblasGemmBatchedExAsync call through the C ABI.  The ctypes
+        argument list is built once per (compute type, stream) and reused, so
+        a step times the library's call, not Python's argument marshalling
+        (about 4 us per call, which would bound the smallest batches)."""
         import ssblasex_amd as S
+        from ssblasex_amd import _lib
 
-        st = S.ssblasGemmBatchedExAsync(self.ta, self.tb, self.m, self.n, self.k, self.alpha, self.pA, self.ab,
-                                        self.lda, self.pB, self.ab, self.ldb, self.beta, self.pC, self.cc,
-                                        self.ldc, self.batch, compute_type, 0, stream)
+        key = (compute_type, stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream or 0))
+        cache = self.__dict__.setdefault("_args", {})
+        if key not in cache:
+            args, keep = S._gemm_args(self.ta, self.tb, self.m, self.n, self.k, self.alpha, self.pA, self.ab,
+                                      self.lda, self.pB, self.ab, self.ldb, self.beta, self.pC, self.cc, self.ldc,
+                                      self.batch, compute_type, 0)
+            cache[key] = (args + (key[1],), keep)
+        st = _lib.load().ssblasGemmBatchedExAsync(*cache[key][0])
         if st != 0:
             raise RuntimeError(f"ssblasGemmBatchedExAsync returned {S.Status(st).name}")
 

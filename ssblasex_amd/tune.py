@@ -122,9 +122,17 @@ def time_config(S, torch, form, ta, tb, m, n, k, batch, cfg, steps, warmup):
     os.environ["SSBLAS_TILE_CONFIG"] = cfg
     stream = torch.cuda.current_stream()
 
+    # the ctypes arguments are built once: Python's marshalling (~4 us a
+    # call) would otherwise bound the small problems and hide the kernels'
+    # differences
+    from . import _lib
+
+    args, keep = S._gemm_args(ta, tb, m, n, k, alpha, pA, ab, ra, pB, ab, rb, beta, pC, cc, m, batch, COMPUTE[form], 0)
+    fn = _lib.load().ssblasGemmBatchedExAsync
+    sh = stream.cuda_stream
+
     def call():
-        st = S.ssblasGemmBatchedExAsync(ta, tb, m, n, k, alpha, pA, ab, ra, pB, ab, rb, beta, pC, cc, m, batch,
-                                        COMPUTE[form], 0, stream)
+        st = fn(*args, sh)
         if st != 0:
             raise RuntimeError(f"status {st}")
 
