@@ -54,19 +54,26 @@ struct GItem {
     int cfg;    // GroupedCfg
 };
 
+// Workgroup items are split into GQ queues, one per XCD: every tile of an
+// instance is in one queue, so the instance's operands are fetched into one
+// XCD's L2 (workgroups are dispatched to the eight XCDs round-robin, so
+// workgroup b runs on XCD b % 8 and takes from queue b % GQ first).
+constexpr int GQ = 8;
+
 struct GroupedDevice {
-    const GItem* items;  // workgroup items, then wave items; each part sorted by estimated cost, longest first
-    int nitems;          // workgroup items
+    const GItem* items;  // workgroup items queue by queue, then wave items; each sorted by estimated cost, longest first
+    int nitems;          // workgroup items (all queues)
     int nwitems;         // wave items (items[nitems .. nitems + nwitems))
+    int qoff[GQ + 1];    // queue q holds items[qoff[q] .. qoff[q + 1])
     const GInst* insts;
     const GGroup* groups;
     const void* const* A;  // the caller's device arrays of device pointers
     const void* const* B;
     void* const* C;
-    // three counters, zero between launches: [0] next workgroup item, [1]
-    // workgroups done, [2] next wave item.  The last workgroup of a launch
-    // resets them, so launches of one plan must be ordered (one stream, or a
-    // graph replayed on one stream).
+    // GQ + 2 counters, zero between launches: [q] next item of queue q,
+    // [GQ] workgroups done, [GQ + 1] next wave item.  The last workgroup of a
+    // launch resets them, so launches of one plan must be ordered (one
+    // stream, or a graph replayed on one stream).
     unsigned* counters;
 };
 

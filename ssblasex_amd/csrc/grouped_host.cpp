@@ -236,7 +236,17 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
 
     // Work lists per family (workgroup items, wave items), longest first.
     const int other_family = fp32_form == K_BF16X3 ? GF_I8_BF16X3 : fp32_form == K_BF16X6 ? GF_I8_BF16X6 : GF_I8_F32;
-    std::vector<HostItem> lists[GF_COUNT], wlists[GF_COUNT];
+    // Workgroup tiles go to the XCD queues instance by instance (all tiles of
+    // an instance in one queue), instances longest first onto the queue with
+    // the least estimated work so far; wave items stay in one list.
+    std::vector<HostItem> qlists[GF_COUNT][GQ], wlists[GF_COUNT];
+    struct InstWork {
+        int f;
+        double ns;
+        size_t begin, end;  // its tiles in `tiles`
+    };
+    std::vector<HostItem> tiles;
+    std::vector<InstWork> work;
     for (long i = 0; i < total; ++i) {
         const GInst& I = insts[(size_t)i];
         if (I.m == 0 || I.n == 0 || I.k == 0) continue;
@@ -245,21 +255,42 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
         const int t = tile_of(cfg);
         const double ns = item_ns(G.form, cfg, I.k, G.beta0 != 0);
         const int f = G.form == K_F64 ? GF_F64 : other_family;
-        auto& L = cfg == GCFG_W32 ? wlists[f] : lists[f];
+        const size_t begin = tiles.size();
         for (int j0 = 0; j0 < I.n; j0 += t)
-            for (int i0 = 0; i0 < I.m; i0 += t) L.push_back(HostItem{GItem{(int)i, i0, j0, cfg}, ns});
+            for (int i0 = 0; i0 < I.m; i0 += t) {
+                const HostItem h{GItem{(int)i, i0, j0, cfg}, ns};
+                if (cfg == GCFG_W32) wlists[f].push_back(h);
+                else tiles.push_back(h);
+            }
+        if (tiles.size() > begin) work.push_back(InstWork{f, ns * (double)(tiles.size() - begin), begin, tiles.size()});
+    }
+    std::vector<size_t> order(work.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work[a].ns > work[b].ns; });
+    double load[GF_COUNT][GQ] = {};
+    for (size_t w : order) {
+        const InstWork& W = work[w];
+        int best = 0;
+        for (int qi = 1; qi < GQ; ++qi)
+            if (load[W.f][qi] < load[W.f][best]) best = qi;
+        load[W.f][best] += W.ns;
+        for (size_t x = W.begin; x < W.end; ++x) qlists[W.f][best].push_back(tiles[x]);
     }
 
     long nitems = 0;
     auto by_cost = [](const HostItem& a, const HostItem& b) { return a.ns > b.ns; };
     for (int f = 0; f < GF_COUNT; ++f) {
-        std::stable_sort(lists[f].begin(), lists[f].end(), by_cost);
+        for (int qi = 0; qi < GQ; ++qi) {
+            std::stable_sort(qlists[f][qi].begin(), qlists[f][qi].end(), by_cost);
+            nitems += (long)qlists[f][qi].size();
+        }
         std::stable_sort(wlists[f].begin(), wlists[f].end(), by_cost);
-        nitems += (long)lists[f].size() + (long)wlists[f].size();
+        nitems += (long)wlists[f].size();
     }
     if (nitems > INT_MAX) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
 
-    // Device image: [counters][groups][instances][items of each family]
+    // Device image: [counters: 16 per family][groups][instances][items of each family]
+    static_assert(GQ + 2 <= 16 && 16 * GF_COUNT * sizeof(unsigned) <= 256, "counters fit the header");
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t off_groups = 256;
     const size_t off_insts = align(off_groups + groups.size() * sizeof(GGroup));
@@ -284,25 +315,31 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
     char* dev = static_cast<char*>(p->mem);
     size_t pos = off_items;
     for (int f = 0; f < GF_COUNT; ++f) {
-        auto& L = lists[f];
+        size_t nq = 0;
+        for (int qi = 0; qi < GQ; ++qi) nq += qlists[f][qi].size();
         auto& WL = wlists[f];
-        if (L.empty() && WL.empty()) continue;
+        if (nq == 0 && WL.empty()) continue;
         GItem* dst = reinterpret_cast<GItem*>(image.data() + pos);
-        for (size_t i = 0; i < L.size(); ++i) dst[i] = L[i].it;
-        for (size_t i = 0; i < WL.size(); ++i) dst[L.size() + i] = WL[i].it;
         GroupedDevice& D = p->fam[p->nfam];
+        size_t x = 0;
+        for (int qi = 0; qi < GQ; ++qi) {
+            D.qoff[qi] = (int)x;
+            for (const HostItem& h : qlists[f][qi]) dst[x++] = h.it;
+        }
+        D.qoff[GQ] = (int)x;
+        for (size_t i = 0; i < WL.size(); ++i) dst[x + i] = WL[i].it;
         D.items = reinterpret_cast<const GItem*>(dev + pos);
-        D.nitems = (int)L.size();
+        D.nitems = (int)nq;
         D.nwitems = (int)WL.size();
         D.insts = reinterpret_cast<const GInst*>(dev + off_insts);
         D.groups = reinterpret_cast<const GGroup*>(dev + off_groups);
         D.A = A;
         D.B = B;
         D.C = C;
-        D.counters = reinterpret_cast<unsigned*>(dev) + 4 * f;
+        D.counters = reinterpret_cast<unsigned*>(dev) + 16 * f;  // GQ + 2 <= 16 per family, 256-byte header
         p->fam_id[p->nfam] = f;
         ++p->nfam;
-        pos += (L.size() + WL.size()) * sizeof(GItem);
+        pos += (nq + WL.size()) * sizeof(GItem);
     }
     e = hipMemcpy(p->mem, image.data(), bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {

@@ -394,8 +394,10 @@ __device__ __forceinline__ TileJob g_job(const GroupedDevice& d, const GItem& it
 }
 
 // FA: the family's first form; FB: its second (== FA for one-form families).
-// Workgroup items first (the longer ones), then wave items: each wave of a
-// workgroup takes W32 items from their own counter until none is left.
+// Workgroup items first (the longer ones), from the workgroup's own XCD
+// queue and, once that is empty, from the other queues in turn; then wave
+// items: each wave of a workgroup takes W32 items from their own counter
+// until none is left.
 template <class FA, class FB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) grouped_gemm(const GroupedDevice d)
 {
@@ -403,38 +405,57 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
     __shared__ __attribute__((aligned(16))) char smem[LA > LB ? LA : LB];
     __shared__ int s_item;
     const int tid = threadIdx.x;
-    if (tid == 0) s_item = (int)atomicAdd(&d.counters[0], 1u);
-    __syncthreads();
-    // wave-uniform, so the item's fields live in scalar registers
-    int it = __builtin_amdgcn_readfirstlane(s_item);
-    __syncthreads();
-    while (it < d.nitems) {
-        // Claim the next item now (its number is read once this one is done,
-        // so the atomic's round trip overlaps this item's work) -- except in
-        // the last two rounds of the queue, where holding a second item would
-        // leave other workgroups idle at the end.
-        const bool early = it + 2 * (int)gridDim.x < d.nitems;
+    int q = blockIdx.x & (GQ - 1), visited = 0;  // this workgroup's XCD queue first
+    // The next item of queue q, moving on to the next queue while q is
+    // empty; -1 once every queue is.  Wave-uniform (read back through LDS).
+    auto claim = [&]() -> int {
+        while (visited < GQ) {
+            if (tid == 0) s_item = (int)atomicAdd(&d.counters[q], 1u);
+            __syncthreads();
+            const int v = __builtin_amdgcn_readfirstlane(s_item);
+            __syncthreads();
+            if (v < d.qoff[q + 1] - d.qoff[q]) return d.qoff[q] + v;
+            q = (q + 1) & (GQ - 1);
+            ++visited;
+        }
+        return -1;
+    };
+    // workgroups that start from one queue
+    const int per_queue = ((int)gridDim.x + GQ - 1) / GQ;
+    int it = claim();
+    while (it >= 0) {
+        // Claim the queue's next item now (its number is read once this one
+        // is done, so the atomic's round trip overlaps this item's work) --
+        // except in the last two rounds of the queue, where holding a second
+        // item would leave other workgroups idle at the end.
+        const int qend = d.qoff[q + 1];
+        const bool early = it + 2 * per_queue < qend;
         unsigned next = 0;
-        if (tid == 0 && early) next = atomicAdd(&d.counters[0], 1u);
+        if (tid == 0 && early) next = atomicAdd(&d.counters[q], 1u);
         const GItem item = d.items[it];
         const GInst in = d.insts[item.inst];
         const GGroup g = d.groups[in.group];
         const TileJob j = g_job(d, item, in, g);
         if (g.form == FA::ID) g_run<FA>(item.cfg, j, in.abytes, in.bbytes, smem);
         else g_run<FB>(item.cfg, j, in.abytes, in.bbytes, smem);
-        if (tid == 0) s_item = (int)(early ? next : atomicAdd(&d.counters[0], 1u));
-        __syncthreads();
-        it = __builtin_amdgcn_readfirstlane(s_item);
-        __syncthreads();
+        if (early) {
+            if (tid == 0) s_item = (int)next;
+            __syncthreads();
+            const int v = __builtin_amdgcn_readfirstlane(s_item);
+            __syncthreads();
+            it = d.qoff[q] + v < qend ? d.qoff[q] + v : claim();
+        } else {
+            it = claim();
+        }
     }
     if (d.nwitems > 0) {
         const int lane = tid & 63;
-        auto claim = [&]() {
+        auto claim_wave = [&]() {
             unsigned v = 0u;
-            if (lane == 0) v = atomicAdd(&d.counters[2], 1u);
+            if (lane == 0) v = atomicAdd(&d.counters[GQ + 1], 1u);
             return __builtin_amdgcn_readfirstlane((int)v);  // lane 0's claim, for the whole wave
         };
-        for (int w = claim(); w < d.nwitems; w = claim()) {
+        for (int w = claim_wave(); w < d.nwitems; w = claim_wave()) {
             const GItem item = d.items[d.nitems + w];
             const GInst in = d.insts[item.inst];
             const GGroup g = d.groups[in.group];
@@ -449,12 +470,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
     // before their fenced arrival).
     if (tid == 0) {
         __threadfence();
-        const unsigned arrived = atomicAdd(&d.counters[1], 1u);
+        const unsigned arrived = atomicAdd(&d.counters[GQ], 1u);
         if (arrived == gridDim.x - 1) {
             __threadfence();
-            atomicExch(&d.counters[0], 0u);
-            atomicExch(&d.counters[2], 0u);
-            atomicExch(&d.counters[1], 0u);
+#pragma unroll
+            for (int c = 0; c < GQ + 2; ++c) atomicExch(&d.counters[c], 0u);
         }
     }
 }
