@@ -413,7 +413,7 @@ CONFIGS = [
     ("c2b1", "int8 TT m=n=k=512 batch 64 beta=1", C2B1, [("native", 0)], 0, 0, True),
     ("c4p", "fp32 TN m=n=k=256 batch 4096 alpha=1.5 beta=-0.5, lda=ldb=ldc=264", C4P, FORMS_C4, 8, 0, False),
     ("c3l", "fp64 NN m=n=k=256 batch 512 U[-1,1]*2^U{-8..8}", C3L,
-     [("native", 0), ("ozaki_s8", 0x108), ("ozaki_s9", 0x109), ("ozaki_auto", 6)], 8, 0, False),
+     [("native", 0)] + [(f"ozaki_s{s}", 0x100 + s) for s in range(3, 10)] + [("ozaki_auto", 6)], 8, 0, False),
 ]
 DEFAULT_FORMS = {"c3": {"native", "ozaki_s9", "ozaki_auto"}}
 

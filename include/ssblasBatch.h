@@ -286,7 +286,7 @@ ssblasStatus_t ssblasGetOzakiReport(
  * Which kernel a call with these arguments would run (no counterpart in the
  * reference; for tests and tuning): writes the tile-config name (e.g.
  * "dma256k2", "tile256x128", "direct16") or the Ozaki kernel family
- * ("ozaki_small", "ozaki_gemm") into name[0 .. name_len-1].  Same status
+ * ("ozaki_small", "ozaki_mma") into name[0 .. name_len-1].  Same status
  * table as ssblasGemmBatchedEx for the arguments it takes; touches no
  * device memory.
  */

@@ -29,13 +29,17 @@ int tuned_config(const GemmProblem& g);
 
 // Workspace of the Ozaki form: the per-instance report (error bound, slice
 // count) at the front -- all the fused form (m, n, k <= 32) needs -- then,
-// for the general path, the int8 slices of op(A) and op(B) (each kept in its
-// operand's storage orientation, leading dimension rounded up to 16 bytes)
-// and the int32 exponent vectors.
+// for the general path, the int32 exponents of op(A)'s rows and op(B)'s
+// columns, their statistics (OZAKI_ROW_BYTES each), and the int8 slices of
+// op(A) and op(B), k-block major: per instance, per k-block of 32, `s`
+// planes of rows_pad x 32 bytes (rows and k padded with zero digits).
+constexpr int OZAKI_ROW_BYTES = 48;
 struct OzakiLayout {
-    long ld_sa, cols_sa, ld_sb, cols_sb;  // slice matrices: storage ld and column count
-    size_t off_bound, off_s;              // float bound[batch], int8 slices[batch]
-    size_t off_a, off_b, off_e, total;    // A's and B's slices, the exponents; total bytes
+    long rows_pad_a, rows_pad_b, nkb;  // m and n rounded up to 64; k-blocks of 32 (k padded to 64)
+    size_t inst_a, inst_b;             // slice bytes per instance
+    size_t off_bound, off_s;           // float bound[batch], int8 slices[batch]
+    size_t off_e, off_r;               // exponents (A, then B), row statistics (A, then B)
+    size_t off_a, off_b, total;        // A's and B's slices; total bytes
 };
 // Where an Ozaki call left its per-instance report (device pointers).
 struct OzakiReportLoc {
@@ -82,7 +86,7 @@ size_t ozaki_workspace_bytes(const GemmProblem& g, int s);
 // The fused one-wave Ozaki form applies (m, n, k <= 32).
 bool ozaki_fused_shape(const GemmProblem& g);
 // Kernel family an Ozaki call of this shape runs ("ozaki_small" for the
-// fused one-wave form, "ozaki_gemm" otherwise).
+// fused one-wave form, "ozaki_mma" otherwise).
 const char* ozaki_kernel_name(const GemmProblem& g);
 // Most slices the automatic choice may take.
 constexpr int OZAKI_MAX_SLICES = 9;

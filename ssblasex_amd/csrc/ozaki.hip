@@ -4,15 +4,14 @@
 //
 // For every instance:
 //   1. e_i = exponent with max_k |op(A)(i,k)| < 2^e_i  (row i of op(A)),
-//      f_j likewise for column j of op(B)               [ozaki_exp_* kernels]
+//      f_j likewise for column j of op(B)
 //   2. A'(i,k) = op(A)(i,k) * 2^-e_i in (-1, 1) is cut into s signed 7-bit
 //      slices:  r = A' ; repeat s times { r *= 128; a_p = trunc(r); r -= a_p }
 //      so A' = sum_p a_p 2^(-7p) + O(2^(-7s)), |a_p| <= 127; same for B.
-//      Every step is exact in fp64.  Slices are stored with the operand's
-//      own orientation and a compact leading dimension [ozaki_slice kernel]
+//      Every step is exact in fp64.
 //   3. for t = s+1 down to 2:  D_t = sum_{p+q=t} a_p b_q  -- exact int32
-//      products on v_mfma_i32_16x16x64_i8 -- and D += D_t * 2^(-7t) in fp64
-//      (smallest terms first)                                [ozaki_gemm]
+//      products on the int8 MFMA -- and D += D_t * 2^(-7t) in fp64
+//      (smallest terms first)
 //   4. C = fma(D * 2^(e_i+f_j), alpha, fl(beta * C))  (the reference's
 //      epilogue rounding order)
 // The triangle p + q <= s + 1 keeps s(s+1)/2 of the s^2 slice products; the
@@ -20,17 +19,28 @@
 // int32 exactness: one D_t sums at most s*kc*127^2 < 2^31 for a K chunk kc
 // of at most 8192; longer K is folded chunk by chunk.
 // Rows or columns holding inf/nan get NaN in every output they touch.
+//
+// Two forms compute the same result bit for bit:
+//   * instances up to 32 x 32 x 32: ozaki_small, one wave per instance,
+//     slices cut in registers;
+//   * larger ones: ozaki_prep (exponents, slices into the workspace, row
+//     statistics; one launch per operand), ozaki_choose (error bound and
+//     slice count per instance), ozaki_mma (slice products and fold).
 #include <hip/hip_runtime.h>
 
 #include <math.h>
 #include <stdlib.h>
 
-#include "gemm_kernels.hpp"
+#include <algorithm>
+
+#include "gemm_dma.hpp"
 #include "api_internal.hpp"
 #include "launch.hpp"
 #include "occupancy.hpp"
 
 namespace ssb {
+
+struct OzRow;
 
 struct OzParams {
     const void* const* A;  // caller's fp64 operands
@@ -39,12 +49,17 @@ struct OzParams {
     long lda, ldb, ldc;
     int m, n, k, batch, s;
     int ta, tb;
-    // workspace
-    int8_t* wa;      // A slices: [inst][p][cols_sa][ld_sa]
-    int8_t* wb;      // B slices
+    // workspace of the general path (see ozaki_prep)
+    int8_t* wa;      // slices of op(A): [inst][kb][plane][rows_pad_a][OZ_KB]
+    int8_t* wb;      // slices of op(B): [inst][kb][plane][rows_pad_b][OZ_KB]
+                     // (kb: k-blocks of OZ_KB, an even count)
     int32_t* ea;     // [inst][m] exponents (INT_MIN marks a non-finite row)
     int32_t* eb;     // [inst][n]
-    long ld_sa, cols_sa, ld_sb, cols_sb;  // slice matrices: storage ld and column count
+    OzRow* ra;       // [inst][m] row statistics of op(A)
+    OzRow* rb;       // [inst][n] column statistics of op(B)
+    long rows_pad_a, rows_pad_b;  // m, n rounded up to OZ_ROWS
+    long inst_a, inst_b;          // slice bytes per instance
+    int nkb, planes;              // k-blocks, slice planes per k-block (= s)
     Scalars sc;
     int beta0;
     int tiles_m, tiles_n;
@@ -58,12 +73,11 @@ struct OzParams {
     int auto_s;
 };
 
+typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
+
 constexpr int OZ_NONFINITE = INT32_MIN;
 constexpr int OZ_KCHUNK = 8192;
 
-// ---------------------------------------------------------------------------
-// exponents
-// ---------------------------------------------------------------------------
 __device__ __forceinline__ int exp_of_max(double mx, bool nonfinite)
 {
     if (nonfinite) return OZ_NONFINITE;
@@ -73,109 +87,267 @@ __device__ __forceinline__ int exp_of_max(double mx, bool nonfinite)
     return e;
 }
 
-// Reduce over the contiguous dimension: one wave per storage column.
-// out[inst * ncols + c] = exponent of max_r |X(r, c)|, r < nrows.
-__global__ void __launch_bounds__(256) ozaki_exp_cols(const void* const* X, long ld, int nrows, int ncols,
-                                                      int32_t* out)
+// A pointer read from a pointer array at a wave-uniform index, as a scalar
+// (the compiler cannot use a scalar load for it: the kernel's stores might
+// alias the array), so element addresses are scalar base + 32-bit offset.
+template <class T>
+__device__ __forceinline__ T* wave_uniform(T* ptr)
 {
-    const int inst = blockIdx.y;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int c = blockIdx.x * 4 + wave;
-    if (c >= ncols) return;
-    const double* x = static_cast<const double*>(X[inst]) + (long)c * ld;
+    const uint64_t v = reinterpret_cast<uint64_t>(ptr);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+// The 7-bit digits of a 28-bit value, one per byte: byte 3 the most
+// significant digit, byte 0 the least.
+__device__ __forceinline__ uint32_t oz_spread7(uint32_t h)
+{
+    return (h & 0x7fu) | ((h << 1) & 0x7f00u) | ((h << 2) & 0x7f0000u) | ((h << 3) & 0x7f000000u);
+}
+
+// ---------------------------------------------------------------------------
+// General path (any dimension above 32).
+//
+// Workspace per instance: the exponents of op(A)'s rows and op(B)'s columns,
+// their statistics (OzRow), and the slices, k-block major: plane p of
+// k-block kb of an operand is rows_pad x OZ_KB bytes, row r's digits of
+// k = OZ_KB kb .. OZ_KB kb + OZ_KB - 1 contiguous:
+//     slices[inst][kb][p][r][kk]
+// ("rows" are op(A)'s rows, or op(B)'s columns).  Rows past the operand and
+// k past K (up to a multiple of 64) hold zero digits, so the product
+// kernel's k-blocks are whole and its loads need no guards.
+// ---------------------------------------------------------------------------
+constexpr int OZ_ROWS = 64;  // rows per prep workgroup and per product tile side
+constexpr int OZ_KB = 32;    // k per block (bytes of one slice row; one MFMA k-step)
+
+// Per row of op(A) (column of op(B)), over k: the scaled values v = x 2^-e
+struct OzRow {
+    float a1, a2;  // sum |v|, sum v^2 (fp32)
+    float d[9];    // sum of digit p + 1 (magnitudes)
+    float depth;   // most digits a value of the row needs (10: more than nine)
+};
+static_assert(sizeof(OzRow) == OZAKI_ROW_BYTES, "row statistics layout");
+
+struct OzPrep {
+    const void* const* X;
+    long ld;
+    int rows, k;      // rows of op(X) (op(B): columns), K
+    int nrb;          // row blocks of OZ_ROWS per instance
+    int planes;       // slice planes written per k-block
+    long rows_pad;    // nrb * OZ_ROWS
+    long inst_bytes;  // slice bytes per instance
+    int8_t* w;
+    int32_t* e;       // [inst][rows]
+    OzRow* st;        // [inst][rows]
+};
+
+// 16 values of row r, k = k0 .. k0 + 15 (zeros past K).  KC: element (r, k)
+// at r * ld + k, else at r + k * ld.  Indices are clamped into the matrix,
+// so every load is in bounds.
+template <bool KC>
+__device__ __forceinline__ void oz_load16(const double* X, long ld, int r, int k0, int K, double (&x)[16])
+{
+    if (KC) {
+        const double* row = X + (long)r * ld;
+        if (k0 + 16 <= K && aligned16(row + k0)) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const f64x2 v = *gbl(reinterpret_cast<const f64x2*>(row + k0) + i);
+                x[2 * i] = v[0];
+                x[2 * i + 1] = v[1];
+            }
+            return;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const double v = *gbl(row + min(k0 + i, K - 1));
+            x[i] = k0 + i < K ? v : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const double v = *gbl(X + r + (long)min(k0 + i, K - 1) * ld);
+            x[i] = k0 + i < K ? v : 0.0;
+        }
+    }
+}
+
+// The nine digits of 16 values of one row (row exponent e) into sl[p]
+// (16 bytes: digit p + 1 of the 16 values, signed), as the sequence
+// "v *= 128; d = trunc(v); v -= d" gives them; done in integers as in the
+// fused kernel (oz_slice_regs).  Row statistics: a1, a2 (fp32 sums of |v|,
+// v^2), dsum[p] (digit magnitudes), rem (a value has digits past the ninth).
+__device__ __forceinline__ void oz_slice16(const double (&x)[16], int e, u32x4 (&sl)[9], float& a1, float& a2,
+                                           uint32_t (&dsum)[9], bool& rem)
+{
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        uint32_t H[4], L[4], N[4], neg = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int i = 4 * j + q;
+            const double a = (e == OZ_NONFINITE) ? 0.0 : ldexp(fabs(x[i]), -e);
+            const float af = (float)a;
+            a1 += af;
+            a2 = __builtin_fmaf(af, af, a2);
+            neg |= (x[i] < 0.0 ? 0xffu : 0u) << (8 * q);
+            const double y1 = ldexp(a, 28);
+            H[q] = (uint32_t)y1;
+            const double y2 = ldexp(y1 - (double)H[q], 28);
+            L[q] = (uint32_t)y2;
+            const double y3 = ldexp(y2 - (double)L[q], 7);
+            N[q] = (uint32_t)y3;
+            rem |= y3 != (double)N[q];
+        }
+        uint32_t w[9];
+        {
+            const uint32_t in[4] = {oz_spread7(H[0]), oz_spread7(H[1]), oz_spread7(H[2]), oz_spread7(H[3])};
+            uint32_t out[4];
+            transpose4x4_bytes(in, out);  // out[c]: byte c of the four values
+            w[0] = out[3];
+            w[1] = out[2];
+            w[2] = out[1];
+            w[3] = out[0];
+        }
+        {
+            const uint32_t in[4] = {oz_spread7(L[0]), oz_spread7(L[1]), oz_spread7(L[2]), oz_spread7(L[3])};
+            uint32_t out[4];
+            transpose4x4_bytes(in, out);
+            w[4] = out[3];
+            w[5] = out[2];
+            w[6] = out[1];
+            w[7] = out[0];
+        }
+        w[8] = N[0] | (N[1] << 8) | (N[2] << 16) | (N[3] << 24);
+#pragma unroll
+        for (int p = 0; p < 9; ++p) {
+            const uint32_t nw = (0x80808080u - w[p]) ^ 0x80808080u;
+            sl[p][j] = (neg & nw) | (~neg & w[p]);
+            dsum[p] = __builtin_amdgcn_sad_u8(w[p], 0u, dsum[p]);
+        }
+    }
+}
+
+// One workgroup per (instance, block of 64 rows): thread t takes row t / 4
+// of the block, k = 16 (t % 4) .. + 15 of every 64 k.  Pass 1: the row
+// maximum -> exponent; pass 2: digits, written as `planes` slice planes,
+// and the row statistics.  Per 64 k and plane the workgroup stores two
+// contiguous 2 KiB k-blocks.  NK > 0 (K <= 64 NK): pass 1 keeps the thread's
+// values in registers for pass 2, so the operand is read once; NK == 0:
+// pass 2 reads it again.
+template <bool KC, int NK>
+__global__ void __launch_bounds__(256) ozaki_prep(const OzPrep q)
+{
+    const int inst = blockIdx.x / q.nrb, rblk = blockIdx.x % q.nrb;
+    const int tid = threadIdx.x, h = tid & 3;
+    const int r = rblk * OZ_ROWS + (tid >> 2);
+    const bool row_ok = r < q.rows;
+    const int rr = row_ok ? r : q.rows - 1;
+    const double* X = static_cast<const double*>(q.X[inst]);
+    const int nk64 = (q.k + 63) / 64;
+
     double mx = 0.0;
     bool bad = false;
-    for (int r = lane; r < nrows; r += 64) {
-        double v = fabs(*gbl(x + r));
-        bad |= !isfinite(v);
-        mx = fmax(mx, v);
+    double xs[NK > 0 ? NK : 1][16];
+    if constexpr (NK > 0) {
+#pragma unroll
+        for (int kb = 0; kb < NK; ++kb) {
+            if (kb < nk64) oz_load16<KC>(X, q.ld, rr, kb * 64 + 16 * h, q.k, xs[kb]);
+        }
+#pragma unroll
+        for (int kb = 0; kb < NK; ++kb) {
+            if (kb < nk64) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const double a = fabs(xs[kb][i]);
+                    bad |= !isfinite(a);
+                    mx = fmax(mx, a);
+                }
+            }
+        }
+    } else {
+        for (int kb = 0; kb < nk64; ++kb) {
+            double x[16];
+            oz_load16<KC>(X, q.ld, rr, kb * 64 + 16 * h, q.k, x);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const double a = fabs(x[i]);
+                bad |= !isfinite(a);
+                mx = fmax(mx, a);
+            }
+        }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
+    for (int off = 1; off < 4; off <<= 1) {
         mx = fmax(mx, __shfl_xor(mx, off));
         bad |= __shfl_xor((int)bad, off) != 0;
     }
-    if (lane == 0) out[(long)inst * ncols + c] = exp_of_max(mx, bad);
-}
+    const int e = row_ok ? exp_of_max(mx, bad) : OZ_NONFINITE;  // padding rows: zero digits
 
-// Reduce over the strided dimension: one thread per storage row, columns
-// split over the 4 waves of the block.
-__global__ void __launch_bounds__(256) ozaki_exp_rows(const void* const* X, long ld, int nrows, int ncols,
-                                                      int32_t* out)
-{
-    __shared__ double smx[4][64];
-    __shared__ int sbad[4][64];
-    const int inst = blockIdx.y;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int r = blockIdx.x * 64 + lane;
-    const double* x = static_cast<const double*>(X[inst]);
-    double mx = 0.0;
-    bool bad = false;
-    if (r < nrows) {
-        for (int c = wave; c < ncols; c += 4) {
-            double v = fabs(*gbl(x + r + (long)c * ld));
-            bad |= !isfinite(v);
-            mx = fmax(mx, v);
-        }
-    }
-    smx[wave][lane] = mx;
-    sbad[wave][lane] = bad;
-    __syncthreads();
-    if (wave == 0 && r < nrows) {
-        mx = fmax(fmax(smx[0][lane], smx[1][lane]), fmax(smx[2][lane], smx[3][lane]));
-        bad = sbad[0][lane] | sbad[1][lane] | sbad[2][lane] | sbad[3][lane];
-        out[(long)inst * nrows + r] = exp_of_max(mx, bad);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// slices: elementwise over the storage of one operand
-// ---------------------------------------------------------------------------
-// exp_by_col: the exponent of element (r, c) is e[c] (else e[r]).
-// s: the most slices (the workspace's plane count); sarr[inst] (when given)
-// the instance's own count: planes past it are not written (nor read).
-__global__ void __launch_bounds__(256) ozaki_slice(const void* const* X, long ld, int nrows, int ncols,
-                                                   const int32_t* e, int exp_by_col, int s, const int8_t* sarr,
-                                                   int8_t* w, long ldw)
-{
-    const int inst = blockIdx.z;
-    const int si = sarr != nullptr ? sarr[inst] : s;
-    const int c = blockIdx.y;
-    const int r0 = (blockIdx.x * 256 + threadIdx.x) * 4;
-    if (r0 >= ldw) return;
-    const double* x = static_cast<const double*>(X[inst]) + (long)c * ld;
-    const int32_t* ei = e + (long)inst * (exp_by_col ? ncols : nrows);
-    const long plane = ldw * ncols;
-    int8_t* out = w + (long)inst * s * plane + (long)c * ldw + r0;
-    double v[4];
+    float a1 = 0.f, a2 = 0.f;
+    uint32_t dsum[9];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int r = r0 + q;
-        v[q] = 0.0;
-        if (r < nrows) {
-            const int ex = exp_by_col ? ei[c] : ei[r];
-            v[q] = (ex == OZ_NONFINITE) ? 0.0 : ldexp(*gbl(x + r), -ex);
+    for (int p = 0; p < 9; ++p) dsum[p] = 0u;
+    bool rem = false;
+    const long plane = q.rows_pad * OZ_KB;
+    // k-block 2 kb + h / 2 of 32, bytes 16 (h % 2) .. of the row
+    int8_t* out = q.w + (long)inst * q.inst_bytes + (h >> 1) * q.planes * plane +
+                  ((long)rblk * OZ_ROWS + (tid >> 2)) * OZ_KB + (h & 1) * 16;
+    auto slice_out = [&](const double(&x)[16], int kb) {
+        u32x4 sl[9];
+        oz_slice16(x, e, sl, a1, a2, dsum, rem);
+        int8_t* o = out + (long)(2 * kb) * q.planes * plane;
+#pragma unroll
+        for (int p = 0; p < 9; ++p)
+            if (p < q.planes) *gbl(reinterpret_cast<u32x4*>(o + p * plane)) = sl[p];
+    };
+    if constexpr (NK > 0) {
+#pragma unroll
+        for (int kb = 0; kb < NK; ++kb)
+            if (kb < nk64) slice_out(xs[kb], kb);
+    } else {
+        for (int kb = 0; kb < nk64; ++kb) {
+            double x[16];
+            oz_load16<KC>(X, q.ld, rr, kb * 64 + 16 * h, q.k, x);
+            slice_out(x, kb);
         }
     }
-    for (int p = 0; p < si; ++p) {
-        uint32_t word = 0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            v[q] *= 128.0;
-            const double t = trunc(v[q]);
-            v[q] -= t;
-            word |= ((uint32_t)(int32_t)t & 0xffu) << (8 * q);
-        }
-        *gbl(reinterpret_cast<uint32_t*>(out + p * plane)) = word;
+    for (int off = 1; off < 4; off <<= 1) {
+        a1 += __shfl_xor(a1, off);
+        a2 += __shfl_xor(a2, off);
+        rem |= __shfl_xor((int)rem, off) != 0;
+#pragma unroll
+        for (int p = 0; p < 9; ++p) dsum[p] += (uint32_t)__shfl_xor((int)dsum[p], off);
+    }
+    if (h == 0 && row_ok) {
+        const long idx = (long)inst * q.rows + r;
+        q.e[idx] = e;
+        int depth = 0;
+#pragma unroll
+        for (int p = 0; p < 9; ++p)
+            if (dsum[p] != 0u) depth = p + 1;
+        OzRow st;
+        st.a1 = a1;
+        st.a2 = a2;
+#pragma unroll
+        for (int p = 0; p < 9; ++p) st.d[p] = (float)dsum[p];
+        st.depth = (float)(rem ? 10 : depth);
+        float4* dst = reinterpret_cast<float4*>(q.st + idx);
+        const float4* src = reinterpret_cast<const float4*>(&st);
+        dst[0] = src[0];
+        dst[1] = src[1];
+        dst[2] = src[2];
     }
 }
 
 // ---------------------------------------------------------------------------
 // per-instance error bound and slice choice of the general path (the bound
 // of oz_bound below, over rows and columns of any count): one workgroup per
-// instance, one pass over op(A) and op(B) with the exponents of the exponent
-// kernels.  Row sums are kept relative to the largest exponent seen so far
-// (rescaled by 4^(old - new) when a larger one comes), so one pass suffices.
+// instance over the row statistics of ozaki_prep.  Row sums are kept
+// relative to the largest exponent seen so far (rescaled by 4^(old - new)
+// when a larger one comes), so one pass suffices.
 // ---------------------------------------------------------------------------
 constexpr int OZ_SMIN = 3, OZ_NCAND = 9 - OZ_SMIN + 1;
 
@@ -237,15 +409,15 @@ __device__ __forceinline__ int oz_block_max(int v, int* red)
     return max(max(red[0], red[1]), max(red[2], red[3]));
 }
 
-__global__ void __launch_bounds__(256) ozaki_stats(const OzParams p)
+__global__ void __launch_bounds__(256) ozaki_choose(const OzParams p)
 {
     __shared__ float redf[4];
     __shared__ int redi[4];
     const int inst = blockIdx.x, tid = threadIdx.x;
-    const double* A = static_cast<const double*>(p.A[inst]);
-    const double* B = static_cast<const double*>(p.B[inst]);
     const int32_t* ea = p.ea + (long)inst * p.m;
     const int32_t* eb = p.eb + (long)inst * p.n;
+    const OzRow* ra = p.ra + (long)inst * p.m;
+    const OzRow* rb = p.rb + (long)inst * p.n;
     const int smax = p.s;
     bool bad = false;
     int Da = 0, Db = 0;  // most digits any value of op(A) / op(B) needs (10: more than nine)
@@ -256,37 +428,21 @@ __global__ void __launch_bounds__(256) ozaki_stats(const OzParams p)
             bad = true;
             continue;
         }
-        float a1 = 0.f, a2 = 0.f, d[9] = {};
-        for (int kk = 0; kk < p.k; ++kk) {
-            const double x = *gbl(A + (p.ta ? (long)kk + (long)i * p.lda : (long)i + (long)kk * p.lda));
-            double v = ldexp(fabs(x), -e);
-            const float vf = (float)v;
-            a1 += vf;
-            a2 = __builtin_fmaf(vf, vf, a2);
-            int depth = 0;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) {  // the slice kernel's digits, magnitudes only
-                v *= 128.0;
-                const double t = trunc(v);
-                v -= t;
-                d[q] += (float)t;
-                if (t != 0.0) depth = q + 1;
-            }
-            Da = max(Da, v != 0.0 ? 10 : depth);
-        }
-        if (a1 == 0.f) continue;  // all-zero row: its products are exact
+        const OzRow r = ra[i];
+        Da = max(Da, (int)r.depth);
+        if (r.a1 == 0.f) continue;  // all-zero row: its products are exact
         R.rescale(e);
         const float w = __builtin_ldexpf(1.f, 2 * max(e - R.emax, -63));
         R.W += w;
-        R.L2 += w * a2;
-        R.A1 += w * a1;
-        R.A1sq += w * a1 * a1;
+        R.L2 += w * r.a2;
+        R.A1 += w * r.a1;
+        R.A1sq += w * r.a1 * r.a1;
         float S = 0.f;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-            S += d[q];
+            S += r.d[q];
             if (q + 1 >= OZ_SMIN) {
-                const float x = S * (1.f / 128.f) + oz_cand_c(q + 1) * a1;
+                const float x = S * (1.f / 128.f) + oz_cand_c(q + 1) * r.a1;
                 R.X1[q + 1 - OZ_SMIN] += w * x;
                 R.X2[q + 1 - OZ_SMIN] += w * x * x;
             }
@@ -299,30 +455,15 @@ __global__ void __launch_bounds__(256) ozaki_stats(const OzParams p)
             bad = true;
             continue;
         }
-        float b1 = 0.f, b2 = 0.f;
-        for (int kk = 0; kk < p.k; ++kk) {
-            const double x = *gbl(B + (p.tb ? (long)j + (long)kk * p.ldb : (long)kk + (long)j * p.ldb));
-            double v = ldexp(fabs(x), -e);
-            const float vf = (float)v;
-            b1 += vf;
-            b2 = __builtin_fmaf(vf, vf, b2);
-            int depth = 0;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) {
-                v *= 128.0;
-                const double t = trunc(v);
-                v -= t;
-                if (t != 0.0) depth = q + 1;
-            }
-            Db = max(Db, v != 0.0 ? 10 : depth);
-        }
-        if (b1 == 0.f) continue;
+        const OzRow r = rb[j];
+        Db = max(Db, (int)r.depth);
+        if (r.a1 == 0.f) continue;
         Cs.rescale(e);
         const float w = __builtin_ldexpf(1.f, 2 * max(e - Cs.emax, -63));
         Cs.W += w;
-        Cs.L2 += w * b2;
-        Cs.B1 += w * b1;
-        Cs.B1sq += w * b1 * b1;
+        Cs.L2 += w * r.a2;
+        Cs.B1 += w * r.a1;
+        Cs.B1sq += w * r.a1 * r.a1;
     }
     bad = oz_block_max(bad ? 1 : 0, redi) != 0;
     Da = oz_block_max(Da, redi);
@@ -384,151 +525,248 @@ __global__ void __launch_bounds__(256) ozaki_stats(const OzParams p)
 }
 
 // ---------------------------------------------------------------------------
-// multi-product int8 GEMM with fp64 fold
+// Slice products with fp64 fold (general path): one 64 x 64 tile of C at a
+// time per workgroup, persistent over the tiles; four waves of 32 x 32, each
+// on v_mfma_i32_32x32x32_i8.  A step is one k-block of 32: every plane of
+// the tile's A rows and B columns (2 KiB each) moves global -> LDS with
+// global_load_lds_dwordx4 into one of four stages.  The loads run three
+// steps ahead through the workgroup's whole sequence of tiles, so a tile's
+// first k-blocks arrive while the previous tile finishes; the tile's row and
+// column exponents ride along with its first step (into LDS, so that the
+// epilogue never waits on the vector memory counter behind later loads).
+// Every slice pair (p, q) of the triangle p + q <= s + 1 is multiplied from
+// LDS into the int32 accumulator of its order t = p + q (s accumulators of
+// 16 registers).  After each K chunk of OZ_KCHUNK the orders are folded
+// into fp64, smallest first (D = fma(D_t, 2^-7t, D) for t = s + 1 .. 2);
+// then the epilogue C = fma(D * 2^(e_i + f_j), alpha, fl(beta * C)).
+// Every step loads all S planes (an instance with fewer slices multiplies
+// only its own), so the steps are equal and the waits fixed counts.
+// LDS rows are 32 bytes (two 16-byte chunks); chunk c of row r sits at chunk
+// c ^ ((r >> 3) & 1), so a fragment read (rows r0 .. r0 + 15 of one chunk
+// column) covers 16 distinct bank groups.
 // ---------------------------------------------------------------------------
-template <bool AKC, bool BKC, int BM, int BN, int BK>
-__global__ void __launch_bounds__(256) ozaki_gemm(const OzParams p)
+template <int S>
+struct OzMma {
+    static constexpr int PLANE = OZ_ROWS * OZ_KB;  // 2 KiB: one plane of the tile's rows
+    static constexpr int STAGE = 2 * S * PLANE;    // A planes, then B planes
+    static constexpr int NST = 4;                  // stages
+    static constexpr int AHEAD = NST - 1;          // steps loaded ahead
+    static constexpr int DPW = S;                  // loads per wave per step (4 S KiB over 4 waves)
+    static constexpr int EXP = NST * STAGE;        // exponents: 4 tiles x (64 rows + 64 columns) int32
+    static constexpr int LDS_BYTES = EXP + 4 * 2 * OZ_ROWS * 4;
+    static_assert(LDS_BYTES <= 160 * 1024, "stages fit the LDS");
+};
+
+__device__ __forceinline__ int oz_chunk(int row, int c) { return row * OZ_KB + ((c ^ ((row >> 3) & 1)) << 4); }
+
+// The workgroup's i-th tile (tiles blockIdx.x, blockIdx.x + gridDim.x, ...;
+// xcd_remap gives one XCD neighbouring tiles).  The host keeps the tile
+// count below 2^31, so the index arithmetic is 32-bit.
+struct OzTile {
+    int inst, i0, j0;
+};
+__device__ __forceinline__ OzTile oz_tile(const OzParams& p, int i)
 {
-    using AR = ArithI8;
-    constexpr int NT = 256, WAVES_M = 2;
-    constexpr int WTM = BM / 2, WTN = BN / 2, FM = WTM / 16, FN = WTN / 16;
-    constexpr int ROWB = BK + 16;
-    constexpr int A_PLANE = BM * ROWB, B_PLANE = BN * ROWB, BUF = A_PLANE + B_PLANE;
-    __shared__ __attribute__((aligned(16))) char smem[2 * BUF];
+    const uint32_t t = blockIdx.x + (uint32_t)i * gridDim.x;
+    const uint32_t total = (uint32_t)p.total;
+    const uint32_t q = total >> 3, r = total & 7, x = t & 7;
+    const uint32_t w = x * q + (x < r ? x : r) + (t >> 3);  // xcd_remap
+    const uint32_t tpi = (uint32_t)(p.tiles_m * p.tiles_n);
+    const uint32_t inst = w / tpi, rem = w - inst * tpi;
+    const uint32_t tm = rem % (uint32_t)p.tiles_m;
+    return OzTile{(int)inst, (int)tm * OZ_ROWS, (int)((rem - tm) / (uint32_t)p.tiles_m) * OZ_ROWS};
+}
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
-    // slice-operand views (int8, compact leading dimensions)
-    const long sra = AKC ? p.ld_sa : 1, ska = AKC ? 1 : p.ld_sa;
-    const long srb = BKC ? p.ld_sb : 1, skb = BKC ? 1 : p.ld_sb;
-    const long plane_a = p.ld_sa * p.cols_sa, plane_b = p.ld_sb * p.cols_sb;
-    const int tiles_per_inst = p.tiles_m * p.tiles_n;
-    const int s = p.s;
+// A byte of a read-only device array through the scalar cache (a vector
+// load would be ordered behind the outstanding LDS loads on the vector
+// memory counter).  The dword holding it is read; the wait is inside.
+__device__ __forceinline__ int oz_scalar_byte(const int8_t* base, int idx)
+{
+    const int8_t* wp = base + (idx & ~3);
+    uint32_t w;
+    __asm__ volatile("s_load_dword %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)" : "=s"(w) : "s"(wp) : "memory");
+    return (int)(int8_t)(w >> (8 * (idx & 3)));
+}
 
-    using SA = Stager<AR, AKC ? 1 : 0, BM, BK, NT>;
-    using SB = Stager<AR, BKC ? 1 : 0, BN, BK, NT>;
-    SA sa;
-    SB sb;
+// Where the loads stand in the workgroup's sequence of steps (step g: tile
+// g / nkb, k-block g % nkb), kept incrementally (all wave-uniform).
+struct OzStream {
+    long g;     // next step to load
+    int kb, i;  // its k-block and tile
+    const char* a;  // slices of that tile's A rows at k-block kb
+    const char* b;  // and of its B columns
+    int inst, i0, j0;
+};
+template <int S>
+__device__ __forceinline__ void oz_stream_tile(const OzParams& p, OzStream& L)
+{
+    const OzTile T = oz_tile(p, L.i);
+    L.inst = T.inst;
+    L.i0 = T.i0;
+    L.j0 = T.j0;
+    L.a = reinterpret_cast<const char*>(p.wa) + T.inst * p.inst_a + (long)T.i0 * OZ_KB;
+    L.b = reinterpret_cast<const char*>(p.wb) + T.inst * p.inst_b + (long)T.j0 * OZ_KB;
+}
 
-    for (long t = blockIdx.x; t < p.total; t += gridDim.x) {
-        const long w = xcd_remap(t, p.total);
-        const int inst = (int)(w / tiles_per_inst);
-        const int rem = (int)(w % tiles_per_inst);
-        const int i0 = (rem % p.tiles_m) * BM;
-        const int j0 = (rem / p.tiles_m) * BN;
-        const int rows_a = min(BM, p.m - i0), rows_b = min(BN, p.n - j0);
-        const char* Ai = reinterpret_cast<const char*>(p.wa) + (long)inst * s * plane_a + (long)i0 * sra;
-        const char* Bi = reinterpret_cast<const char*>(p.wb) + (long)inst * s * plane_b + (long)j0 * srb;
-        const int si = p.sarr != nullptr ? p.sarr[inst] : s;  // this instance's slice count
-        const bool a_vec = (p.ld_sa % SA::word_bytes(AKC)) == 0, b_vec = (p.ld_sb % SB::word_bytes(BKC)) == 0;
-        const bool a_fast = a_vec && rows_a == BM, b_fast = b_vec && rows_b == BN;
+// Loads of step L.g into stage g % NST, then advance.  1 KiB block blk of
+// the stage (blk = wave + 4 j): plane blk / 2 (A planes first), rows
+// 32 (blk % 2) .. + 31; lane l fills row 32 (blk % 2) + l / 2, LDS chunk
+// l % 2, with the source chunk that belongs there.  A tile's first step
+// also brings its exponents (waves 0 and 1).
+template <int S>
+__device__ __forceinline__ void oz_issue(const OzParams& p, OzStream& L, long gend, char* smem, int wave, int lane)
+{
+    using M = OzMma<S>;
+    const long plane_a = p.rows_pad_a * OZ_KB, plane_b = p.rows_pad_b * OZ_KB;
+    char* st = smem + (int)(L.g & (M::NST - 1)) * M::STAGE;
+    const int src = oz_chunk(((wave & 1) << 5) + (lane >> 1), lane & 1);  // blk & 1 == wave & 1
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+        const int blk = wave + 4 * j;  // 0 .. 4 S - 1
+        const int pl = blk >> 1;       // 0 .. 2 S - 1
+        const char* from = pl < S ? L.a + pl * plane_a + src : L.b + (pl - S) * plane_b + src;
+        __builtin_amdgcn_global_load_lds(from, (lds_void_t*)(st + blk * 1024), 16, 0, 0);
+    }
+    if (L.kb == 0 && wave < 2) {
+        // row (wave 0) or column (wave 1) exponents of the tile; lanes past
+        // the operand read its last one
+        const int32_t* e = wave == 0 ? p.ea + (long)L.inst * p.m + min(L.i0 + lane, p.m - 1)
+                                     : p.eb + (long)L.inst * p.n + min(L.j0 + lane, p.n - 1);
+        __builtin_amdgcn_global_load_lds(e, (lds_void_t*)(smem + M::EXP + ((L.i & 3) * 2 + wave) * OZ_ROWS * 4), 4,
+                                         0, 0);
+    }
+    ++L.g;
+    L.a += S * plane_a;
+    L.b += S * plane_b;
+    if (++L.kb == p.nkb) {
+        L.kb = 0;
+        ++L.i;
+        if (L.g < gend) oz_stream_tile<S>(p, L);
+    }
+}
 
-        f64x4 D[FM][FN];
-#pragma unroll
-        for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < FN; ++fn) D[fm][fn] = f64x4{0.0, 0.0, 0.0, 0.0};
+// One tile: steps g .. g + nkb - 1 of the workgroup's sequence (gend steps
+// in all).  FULL: the instance takes all S slices (no per-pair test).
+template <int S, bool FULL>
+__device__ __forceinline__ void oz_mma_tile(const OzParams& p, const OzTile& T, int i, int si, char* smem, long& g,
+                                            OzStream& L, long gend)
+{
+    using M = OzMma<S>;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 1, wn = wave >> 1;
+    const int fr_a = wm * 32 + (lane & 31), fr_b = wn * 32 + (lane & 31), kh = lane >> 5;
+    const int nkb = p.nkb;
 
-        for (int kc0 = 0; kc0 < p.k; kc0 += OZ_KCHUNK) {
-            const int kc = min(OZ_KCHUNK, p.k - kc0);
-            const int nk = (kc + BK - 1) / BK;
-            for (int tt = si + 1; tt >= 2; --tt) {
-                const int plo = max(1, tt - si), phi = min(si, tt - 1);
-                const int nsteps = (phi - plo + 1) * nk;
-                i32x4 acc[FM][FN];
+    double D[16];
 #pragma unroll
-                for (int fm = 0; fm < FM; ++fm)
+    for (int v = 0; v < 16; ++v) D[v] = 0.0;
+    // K chunks: the accumulators are only updated by the MFMAs inside the
+    // k-block loop (a fold inside it would make the compiler move all of
+    // them between accumulation and ordinary registers every k-block)
+    for (int c0 = 0; c0 < nkb; c0 += OZ_KCHUNK / OZ_KB) {
+        const int c1 = min(nkb, c0 + OZ_KCHUNK / OZ_KB);
+        i32x16 acc[S];
 #pragma unroll
-                    for (int fn = 0; fn < FN; ++fn) acc[fm][fn] = i32x4{0, 0, 0, 0};
-
-                auto load_step = [&](int u) {
-                    const int pa = plo + u / nk, pb = tt - pa;
-                    const int k0 = kc0 + (u % nk) * BK;
-                    const int kv = p.k - k0;
-                    const char* ak = Ai + (long)(pa - 1) * plane_a + (long)k0 * ska;
-                    const char* bk = Bi + (long)(pb - 1) * plane_b + (long)k0 * skb;
-                    const int kv_c = min(kv, kc0 + kc - k0);
-                    if (a_fast && kv_c >= BK) sa.load_fast(AKC, ak, sra, ska, tid);
-                    else sa.load_guarded(AKC, ak, sra, ska, rows_a, kv_c, a_vec, tid);
-                    if (b_fast && kv_c >= BK) sb.load_fast(BKC, bk, srb, skb, tid);
-                    else sb.load_guarded(BKC, bk, srb, skb, rows_b, kv_c, b_vec, tid);
-                };
-                auto store_step = [&](int buf) {
-                    char* base = smem + buf * BUF;
-                    sa.store(AKC, base, A_PLANE, ROWB, tid);
-                    sb.store(BKC, base + A_PLANE, B_PLANE, ROWB, tid);
-                };
-
-                load_step(0);
-                __syncthreads();
-                store_step(0);
-                __syncthreads();
-                for (int u = 0; u < nsteps; ++u) {
-                    const int buf = u & 1;
-                    if (u + 1 < nsteps) load_step(u + 1);
-                    const char* la = smem + buf * BUF;
-                    const char* lb = la + A_PLANE;
+        for (int t = 0; t < S; ++t) acc[t] = i32x16{};
+        for (int kb = c0; kb < c1; ++kb, ++g) {
+            // step g's loads are complete once at most the later steps'
+            // remain in flight (loads and stores retire in order; waiting
+            // for more than needed is safe)
+            if (g + M::AHEAD - 1 < gend) wait_vm<(M::AHEAD - 1) * M::DPW>();
+            else wait_vm<0>();
+            lds_barrier();
+            if (L.g < gend) oz_issue<S>(p, L, gend, smem, wave, lane);  // step g + AHEAD
+            const char* la = smem + (int)(g % M::NST) * M::STAGE;
+            const char* lb = la + S * M::PLANE;
+            u32x4 fa[S], fb[S];
 #pragma unroll
-                    for (int ks = 0; ks < BK / AR::KSTEP; ++ks) {
-                        const int kbyte = ks * AR::KSTEP + (lane >> 4) * 16;
-                        u32x4 af[FM], bfg[FN];
-#pragma unroll
-                        for (int fm = 0; fm < FM; ++fm)
-                            af[fm] = *reinterpret_cast<const u32x4*>(la + (wm * WTM + fm * 16 + (lane & 15)) * ROWB + kbyte);
-#pragma unroll
-                        for (int fn = 0; fn < FN; ++fn)
-                            bfg[fn] = *reinterpret_cast<const u32x4*>(lb + (wn * WTN + fn * 16 + (lane & 15)) * ROWB + kbyte);
-#pragma unroll
-                        for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-                            for (int fn = 0; fn < FN; ++fn) AR::mma(acc[fm][fn], &af[fm], &bfg[fn]);
-                    }
-                    if (u + 1 < nsteps) store_step(buf ^ 1);
-                    __syncthreads();
+            for (int q = 0; q < S; ++q) {
+                if (FULL || q < si) {
+                    fa[q] = *reinterpret_cast<const u32x4*>(la + q * M::PLANE + oz_chunk(fr_a, kh));
+                    fb[q] = *reinterpret_cast<const u32x4*>(lb + q * M::PLANE + oz_chunk(fr_b, kh));
                 }
-                // fold: D += D_t * 2^(-7 t)
-                const double scale = ldexp(1.0, -7 * tt);
+            }
 #pragma unroll
-                for (int fm = 0; fm < FM; ++fm)
+            for (int pa = 1; pa <= S; ++pa)
 #pragma unroll
-                    for (int fn = 0; fn < FN; ++fn) {
-                        int32_t a4[4];
-                        __builtin_memcpy(a4, &acc[fm][fn], 16);
-                        double d4[4];
-                        __builtin_memcpy(d4, &D[fm][fn], 32);
+                for (int qb = 1; pa + qb <= S + 1; ++qb)
+                    if (FULL || pa + qb <= si + 1)
+                        acc[pa + qb - 2] = __builtin_amdgcn_mfma_i32_32x32x32_i8((i32x4)fa[pa - 1], (i32x4)fb[qb - 1],
+                                                                                acc[pa + qb - 2], 0, 0, 0);
+        }
+        // fold: orders above si + 1 are zero and leave D unchanged
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) d4[v] = __builtin_fma((double)a4[v], scale, d4[v]);
-                        __builtin_memcpy(&D[fm][fn], d4, 32);
-                    }
+        for (int t = S + 1; t >= 2; --t) {
+            const double scale = ldexp(1.0, -7 * t);
+#pragma unroll
+            for (int v = 0; v < 16; ++v) D[v] = __builtin_fma((double)acc[t - 2][v], scale, D[v]);
+        }
+    }
+
+    // epilogue: lane -> column (lane & 31), accumulator v -> row
+    // 8 (v / 4) + 4 (lane / 32) + v % 4 of the wave's 32 x 32 block
+    const int c = T.j0 + wn * 32 + (lane & 31);
+    if (c >= p.n) return;
+    const int32_t* ex = reinterpret_cast<const int32_t*>(smem + M::EXP + (int)(i & 3) * 2 * OZ_ROWS * 4);
+    const int fj = ex[OZ_ROWS + wn * 32 + (lane & 31)];
+    double* C = wave_uniform(static_cast<double*>(p.C[T.inst]));
+    const bool beta0 = p.beta0 != 0;
+    const bool vec = (p.ldc % 2) == 0 && aligned16(C);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r0 = wm * 32 + 8 * q + 4 * kh;  // row within the tile
+        const int row0 = T.i0 + r0;
+        if (row0 >= p.m) continue;
+        double val[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int ei = ex[r0 + v];
+            val[v] = (ei == OZ_NONFINITE || fj == OZ_NONFINITE) ? __builtin_nan("") : ldexp(D[4 * q + v], ei + fj);
+        }
+        double* dst = C + (long)c * p.ldc + row0;
+        if (vec && row0 + 4 <= p.m) {
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                f64x2 old = {0.0, 0.0};
+                if (!beta0) old = *gbl(reinterpret_cast<const f64x2*>(dst + 2 * hh));
+                f64x2 res;
+                res[0] = __builtin_fma(val[2 * hh], p.sc.ad, beta0 ? 0.0 : old[0] * p.sc.bd);
+                res[1] = __builtin_fma(val[2 * hh + 1], p.sc.ad, beta0 ? 0.0 : old[1] * p.sc.bd);
+                *gbl(reinterpret_cast<f64x2*>(dst + 2 * hh)) = res;
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (row0 + v < p.m) {
+                    const double t0 = beta0 ? 0.0 : (*gbl(dst + v)) * p.sc.bd;
+                    *gbl(dst + v) = __builtin_fma(val[v], p.sc.ad, t0);
+                }
             }
         }
+    }
+}
 
-        // epilogue: C = fma(D * 2^(e_i + f_j), alpha, fl(beta * C))
-        double* C = static_cast<double*>(p.C[inst]);
-        const int32_t* ea = p.ea + (long)inst * p.m;
-        const int32_t* eb = p.eb + (long)inst * p.n;
-        const bool beta0 = p.beta0 != 0;
+template <int S>
+__global__ void __launch_bounds__(256) ozaki_mma(const OzParams p)
+{
+    using M = OzMma<S>;
+    __shared__ __attribute__((aligned(1024))) char smem[M::LDS_BYTES];
+    const int ntiles = p.total > (long)blockIdx.x ? (int)((p.total - 1 - blockIdx.x) / gridDim.x) + 1 : 0;
+    const long gend = (long)ntiles * p.nkb;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    OzStream L{0, 0, 0, nullptr, nullptr, 0, 0, 0};
+    if (gend > 0) oz_stream_tile<S>(p, L);
 #pragma unroll
-        for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < FN; ++fn) {
-                const int r0 = i0 + wm * WTM + fm * 16 + 4 * (lane >> 4);
-                const int c = j0 + wn * WTN + fn * 16 + (lane & 15);
-                if (c >= p.n) continue;
-                const int fj = eb[c];
-                double d4[4];
-                __builtin_memcpy(d4, &D[fm][fn], 32);
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int r = r0 + v;
-                    if (r >= p.m) break;
-                    const int ei = ea[r];
-                    double val = (ei == OZ_NONFINITE || fj == OZ_NONFINITE) ? __builtin_nan("") : ldexp(d4[v], ei + fj);
-                    double* dst = C + (long)c * p.ldc + r;
-                    double t0 = beta0 ? 0.0 : (*gbl(dst)) * p.sc.bd;
-                    *gbl(dst) = __builtin_fma(val, p.sc.ad, t0);
-                }
-            }
+    for (int q = 0; q < M::AHEAD; ++q)
+        if (L.g < gend) oz_issue<S>(p, L, gend, smem, wave, lane);
+    long g = 0;
+    for (int i = 0; i < ntiles; ++i) {
+        const OzTile T = oz_tile(p, i);
+        const int si = oz_scalar_byte(p.sarr, T.inst);
+        if (si >= S) oz_mma_tile<S, true>(p, T, i, S, smem, g, L, gend);
+        else oz_mma_tile<S, false>(p, T, i, si, smem, g, L, gend);
     }
 }
 
@@ -548,21 +786,13 @@ __global__ void __launch_bounds__(256) ozaki_gemm(const OzParams p)
 // is lane l -> column l % 32, accumulator v -> row 8*(v/4) + 4*(l/32) + v%4.
 // D_t = sum of the pairs of order t is exact in int32 (9 * 32 * 127^2 <
 // 2^31); the fold (smallest order first), the exponents and the epilogue
-// are ozaki_gemm's, so the result is bit-identical to the general path.
+// are ozaki_mma's, so the result is bit-identical to the general path.
 // ---------------------------------------------------------------------------
-typedef int32_t i32x16 __attribute__((ext_vector_type(16)));
-
-// The 7-bit digits of a 28-bit value, one per byte: byte 3 the most
-// significant digit, byte 0 the least.
-__device__ __forceinline__ uint32_t oz_spread7(uint32_t h)
-{
-    return (h & 0x7fu) | ((h << 1) & 0x7f00u) | ((h << 2) & 0x7f0000u) | ((h << 3) & 0x7f000000u);
-}
 
 // Slices of 16 values (v = x * 2^-e, |v| < 1) into S 16-byte chunks: slice
 // p holds, for every value, its p-th base-128 digit with the value's sign
 // (the digits the sequence "v *= 128; d = trunc(v); v -= d" gives, as in
-// ozaki_slice).  Done in integers: H = floor(|v| 2^28) carries digits 1-4,
+// step 2 above).  Done in integers: H = floor(|v| 2^28) carries digits 1-4,
 // L = floor(frac(|v| 2^28) 2^28) digits 5-8 and floor(frac(|v| 2^56) 2^7)
 // digit 9 (every step exact in fp64); the 7-bit fields are spread to bytes
 // and transposed across 4 values; a negative value's bytes are negated at
@@ -705,7 +935,7 @@ __device__ __forceinline__ int oz_max32(int v)
 // both halves are joined first).  The fused kernel's time is set by its
 // loads and its slicing, not by the slice count, so its automatic choice is
 // always the most slices and only the bound at S is evaluated here; the
-// general path chooses per instance (ozaki_stats).
+// general path chooses per instance (ozaki_choose).
 template <int S>
 __device__ __forceinline__ float oz_bound(int ea, int eb, OzStats sa, OzStats sb, const uint32_t (&dsum)[S])
 {
@@ -772,17 +1002,6 @@ __device__ __forceinline__ OzLane oz_lane(long ld, bool kcontig, int rows, int k
     L.kvalid = (uint32_t)max(0, min(16, k - 16 * kh));
     L.row_ok = own < rows;
     return L;
-}
-// A pointer read from a pointer array at a wave-uniform index, as a scalar
-// (the compiler cannot use a scalar load for it: the kernel's stores might
-// alias the array), so element addresses are scalar base + 32-bit offset.
-template <class T>
-__device__ __forceinline__ T* wave_uniform(T* ptr)
-{
-    const uint64_t v = reinterpret_cast<uint64_t>(ptr);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
 }
 
 // FULL: the instance is a whole 32 x 32 x 32 block (no element to zero).
@@ -963,41 +1182,76 @@ static hipError_t ozaki_small_dispatch(const OzParams& p, hipStream_t st)
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-template <bool AKC, bool BKC>
-static hipError_t run_ozaki_gemm(OzParams p, hipStream_t st)
+template <int S>
+static hipError_t run_ozaki_mma_s(const OzParams& p0, hipStream_t st)
 {
-    constexpr int BM = 64, BN = 64, BK = 64;
-    auto kern = ozaki_gemm<AKC, BKC, BM, BN, BK>;
-    p.tiles_m = (p.m + BM - 1) / BM;
-    p.tiles_n = (p.n + BN - 1) / BN;
-    p.total = (long)p.batch * p.tiles_m * p.tiles_n;
+    auto kern = ozaki_mma<S>;
     const int occ = kernel_occupancy(kern, 256);
-    long slots = (long)occ * device_cu_count();
-    long grid = p.total <= slots ? p.total : (slots / 8) * 8;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, p);
-    return hipGetLastError();
-}
-
-// Exponents of one operand's rows (op(A)) or columns (op(B)).  Storage is
-// nrows x ncols (column major); exponents are per storage column when
-// exp_by_col.
-static hipError_t exp_operand(const void* const* X, long ld, int nrows, int ncols, int exp_by_col, int batch,
-                              int32_t* e, hipStream_t st)
-{
-    if (exp_by_col) {
-        hipLaunchKernelGGL(ozaki_exp_cols, dim3((ncols + 3) / 4, batch), dim3(256), 0, st, X, ld, nrows, ncols, e);
-    } else {
-        hipLaunchKernelGGL(ozaki_exp_rows, dim3((nrows + 63) / 64, batch), dim3(256), 0, st, X, ld, nrows, ncols, e);
+    const long slots = (long)occ * device_cu_count();
+    OzParams p = p0;
+    p.tiles_m = (p.m + OZ_ROWS - 1) / OZ_ROWS;
+    p.tiles_n = (p.n + OZ_ROWS - 1) / OZ_ROWS;
+    const long tpi = (long)p.tiles_m * p.tiles_n;
+    // the kernel's tile arithmetic is 32-bit: launches of fewer than 2^31 tiles
+    const long per_launch = ((1L << 31) - 1) / tpi;
+    for (long b0 = 0; b0 < p0.batch; b0 += per_launch) {
+        p.batch = (int)std::min<long>(per_launch, p0.batch - b0);
+        p.A = p0.A + b0;
+        p.B = p0.B + b0;
+        p.C = p0.C + b0;
+        p.wa = p0.wa + b0 * p0.inst_a;
+        p.wb = p0.wb + b0 * p0.inst_b;
+        p.ea = p0.ea + b0 * p0.m;
+        p.eb = p0.eb + b0 * p0.n;
+        p.sarr = p0.sarr + b0;
+        p.total = (long)p.batch * tpi;
+        // a multiple of 8 workgroups keeps workgroup b on XCD b % 8 for every
+        // tile it takes (xcd_remap)
+        const long grid = p.total <= slots ? p.total : (slots / 8) * 8;
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, p);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
-    return hipGetLastError();
+    return hipSuccess;
 }
 
-static hipError_t slice_operand(const void* const* X, long ld, int nrows, int ncols, int exp_by_col, int s,
-                                const int8_t* sarr, int batch, const int32_t* e, int8_t* w, long ldw,
-                                hipStream_t st)
+static hipError_t run_ozaki_mma(const OzParams& p, hipStream_t st)
 {
-    hipLaunchKernelGGL(ozaki_slice, dim3((unsigned)((ldw / 4 + 255) / 256), ncols, batch), dim3(256), 0, st, X, ld,
-                       nrows, ncols, e, exp_by_col, s, sarr, w, ldw);
+    switch (p.planes) {
+    case 3: return run_ozaki_mma_s<3>(p, st);
+    case 4: return run_ozaki_mma_s<4>(p, st);
+    case 5: return run_ozaki_mma_s<5>(p, st);
+    case 6: return run_ozaki_mma_s<6>(p, st);
+    case 7: return run_ozaki_mma_s<7>(p, st);
+    case 8: return run_ozaki_mma_s<8>(p, st);
+    default: return run_ozaki_mma_s<9>(p, st);
+    }
+}
+
+// Exponents, slices and statistics of one operand's rows (op(A)) or columns
+// (op(B)).  kc: element (row, k) at row * ld + k.
+static hipError_t prep_operand(const void* const* X, long ld, int rows, int k, bool kc, int batch, int planes,
+                               long rows_pad, long inst_bytes, int8_t* w, int32_t* e, OzRow* stats, hipStream_t st)
+{
+    OzPrep q;
+    q.X = X;
+    q.ld = ld;
+    q.rows = rows;
+    q.k = k;
+    q.nrb = (int)(rows_pad / OZ_ROWS);
+    q.planes = planes;
+    q.rows_pad = rows_pad;
+    q.inst_bytes = inst_bytes;
+    q.w = w;
+    q.e = e;
+    q.st = stats;
+    const dim3 grid((unsigned)((long)batch * q.nrb));
+    // the operand stays in registers between the passes up to K = 256
+    auto launch = [&](auto kern) { hipLaunchKernelGGL(kern, grid, dim3(256), 0, st, q); };
+    if (k <= 64) kc ? launch(ozaki_prep<true, 1>) : launch(ozaki_prep<false, 1>);
+    else if (k <= 128) kc ? launch(ozaki_prep<true, 2>) : launch(ozaki_prep<false, 2>);
+    else if (k <= 256) kc ? launch(ozaki_prep<true, 4>) : launch(ozaki_prep<false, 4>);
+    else kc ? launch(ozaki_prep<true, 0>) : launch(ozaki_prep<false, 0>);
     return hipGetLastError();
 }
 
@@ -1042,37 +1296,30 @@ hipError_t ozaki_run(const GemmProblem& g, int slices, WsGet ws_get, hipStream_t
 
     if (ozaki_fused_shape(g)) return ozaki_small_dispatch(p, st);  // slices in registers: no more workspace
 
-    const long ra = g.ta ? k : m, ca = g.ta ? m : k;  // storage of A
-    const long rb = g.tb ? n : k, cb = g.tb ? k : n;  // storage of B
     p.wa = reinterpret_cast<int8_t*>(ws + L.off_a);
     p.wb = reinterpret_cast<int8_t*>(ws + L.off_b);
     p.ea = reinterpret_cast<int32_t*>(ws + L.off_e);
     p.eb = p.ea + (size_t)g.batch * m;
-    p.ld_sa = L.ld_sa;
-    p.cols_sa = ca;
-    p.ld_sb = L.ld_sb;
-    p.cols_sb = cb;
+    p.ra = reinterpret_cast<OzRow*>(ws + L.off_r);
+    p.rb = p.ra + (size_t)g.batch * m;
+    p.rows_pad_a = L.rows_pad_a;
+    p.rows_pad_b = L.rows_pad_b;
+    p.inst_a = (long)L.inst_a;
+    p.inst_b = (long)L.inst_b;
+    p.nkb = (int)L.nkb;
+    p.planes = s;
 
-    // op(A) row i: storage row i (N) -> reduce along the strided dim; storage column i (T)
-    err = exp_operand(g.A, g.lda, (int)ra, (int)ca, g.ta ? 1 : 0, g.batch, p.ea, st);
+    // op(A) row i is K-contiguous when A is transposed; op(B) column j when
+    // B is not
+    err = prep_operand(g.A, g.lda, m, k, g.ta != 0, g.batch, s, L.rows_pad_a, (long)L.inst_a, p.wa, p.ea, p.ra, st);
     if (err != hipSuccess) return err;
-    // op(B) column j: storage column j (N); storage row j (T)
-    err = exp_operand(g.B, g.ldb, (int)rb, (int)cb, g.tb ? 0 : 1, g.batch, p.eb, st);
+    err = prep_operand(g.B, g.ldb, n, k, g.tb == 0, g.batch, s, L.rows_pad_b, (long)L.inst_b, p.wb, p.eb, p.rb, st);
     if (err != hipSuccess) return err;
     // error bound and slice count per instance
-    hipLaunchKernelGGL(ozaki_stats, dim3((unsigned)g.batch), dim3(256), 0, st, p);
+    hipLaunchKernelGGL(ozaki_choose, dim3((unsigned)g.batch), dim3(256), 0, st, p);
     err = hipGetLastError();
     if (err != hipSuccess) return err;
-    err = slice_operand(g.A, g.lda, (int)ra, (int)ca, g.ta ? 1 : 0, s, p.sarr, g.batch, p.ea, p.wa, L.ld_sa, st);
-    if (err != hipSuccess) return err;
-    err = slice_operand(g.B, g.ldb, (int)rb, (int)cb, g.tb ? 0 : 1, s, p.sarr, g.batch, p.eb, p.wb, L.ld_sb, st);
-    if (err != hipSuccess) return err;
-
-    // slice operand views: A-T / B-N are K-contiguous
-    if (!g.ta && !g.tb) return run_ozaki_gemm<false, true>(p, st);
-    if (!g.ta && g.tb) return run_ozaki_gemm<false, false>(p, st);
-    if (g.ta && !g.tb) return run_ozaki_gemm<true, true>(p, st);
-    return run_ozaki_gemm<true, false>(p, st);
+    return run_ozaki_mma(p, st);
 }
 
 }  // namespace ssb

@@ -66,27 +66,25 @@ bool ozaki_fused_shape(const GemmProblem& g)
 
 OzakiLayout ozaki_layout(const GemmProblem& g, int s)
 {
-    OzakiLayout L;
+    OzakiLayout L{};
     const long m = g.m, n = g.n, k = g.k;
-    const long ra = g.ta ? k : m, ca = g.ta ? m : k;  // storage of A
-    const long rb = g.tb ? n : k, cb = g.tb ? k : n;  // storage of B
-    L.ld_sa = (ra + 15) / 16 * 16;
-    L.ld_sb = (rb + 15) / 16 * 16;
-    L.cols_sa = ca;
-    L.cols_sb = cb;
+    const size_t batch = (size_t)g.batch;
+    L.rows_pad_a = (m + 63) / 64 * 64;
+    L.rows_pad_b = (n + 63) / 64 * 64;
+    L.nkb = (k + 63) / 64 * 2;  // k-blocks of 32, k padded to a multiple of 64
+    L.inst_a = (size_t)L.nkb * s * L.rows_pad_a * 32;
+    L.inst_b = (size_t)L.nkb * s * L.rows_pad_b * 32;
     L.off_bound = 0;
-    L.off_s = align256((size_t)g.batch * sizeof(float));
-    L.off_a = L.off_s + align256((size_t)g.batch);
+    L.off_s = align256(batch * sizeof(float));
+    L.off_e = L.off_s + align256(batch);
     if (ozaki_fused_shape(g)) {
-        L.off_b = L.off_e = L.total = L.off_a;
+        L.off_r = L.off_a = L.off_b = L.total = L.off_e;
         return L;
     }
-    const size_t bytes_a = (size_t)g.batch * s * L.ld_sa * ca;
-    const size_t bytes_b = (size_t)g.batch * s * L.ld_sb * cb;
-    const size_t bytes_e = (size_t)g.batch * (m + n) * sizeof(int32_t);
-    L.off_b = L.off_a + align256(bytes_a);
-    L.off_e = L.off_b + align256(bytes_b);
-    L.total = L.off_e + bytes_e;
+    L.off_r = L.off_e + align256(batch * (m + n) * sizeof(int32_t));
+    L.off_a = L.off_r + align256(batch * (m + n) * OZAKI_ROW_BYTES);
+    L.off_b = L.off_a + align256(batch * L.inst_a);
+    L.total = L.off_b + batch * L.inst_b;
     return L;
 }
 
@@ -97,7 +95,7 @@ size_t ozaki_workspace_bytes(const GemmProblem& g, int s)
 
 const char* ozaki_kernel_name(const GemmProblem& g)
 {
-    return ozaki_fused_shape(g) ? "ozaki_small" : "ozaki_gemm";
+    return ozaki_fused_shape(g) ? "ozaki_small" : "ozaki_mma";
 }
 
 hipError_t launch_ozaki(const GemmProblem& g, int slices, hipStream_t stream, WsGet ws_get, ssblasStatus_t* st)
