@@ -35,7 +35,7 @@ int tuned_config(const GemmProblem& g);
 // planes of rows_pad x 32 bytes (rows and k padded with zero digits).
 constexpr int OZAKI_ROW_BYTES = 48;
 struct OzakiLayout {
-    long rows_pad_a, rows_pad_b, nkb;  // m and n rounded up to 64; k-blocks of 32 (k padded to 64)
+    long rows_pad_a, rows_pad_b, nkb;  // m rounded up to the tile rows, n to 64; k-blocks of 32 (k padded to 64)
     size_t inst_a, inst_b;             // slice bytes per instance
     size_t off_bound, off_s;           // float bound[batch], int8 slices[batch]
     size_t off_e, off_r;               // exponents (A, then B), row statistics (A, then B)
@@ -88,6 +88,9 @@ bool ozaki_fused_shape(const GemmProblem& g);
 // Kernel family an Ozaki call of this shape runs ("ozaki_small" for the
 // fused one-wave form, "ozaki_mma" otherwise).
 const char* ozaki_kernel_name(const GemmProblem& g);
+// Rows of op(A) per tile of the general Ozaki product kernel (op(A)'s slices
+// are padded to a multiple of it).
+inline int ozaki_tile_rows(long m) { return m > 64 ? 128 : 64; }
 // Most slices the automatic choice may take.
 constexpr int OZAKI_MAX_SLICES = 9;
 

@@ -525,35 +525,43 @@ __global__ void __launch_bounds__(256) ozaki_choose(const OzParams p)
 }
 
 // ---------------------------------------------------------------------------
-// Slice products with fp64 fold (general path): one 64 x 64 tile of C at a
-// time per workgroup, persistent over the tiles; four waves of 32 x 32, each
-// on v_mfma_i32_32x32x32_i8.  A step is one k-block of 32: every plane of
-// the tile's A rows and B columns (2 KiB each) moves global -> LDS with
-// global_load_lds_dwordx4 into one of four stages.  The loads run three
-// steps ahead through the workgroup's whole sequence of tiles, so a tile's
-// first k-blocks arrive while the previous tile finishes; the tile's row and
-// column exponents ride along with its first step (into LDS, so that the
-// epilogue never waits on the vector memory counter behind later loads).
-// Every slice pair (p, q) of the triangle p + q <= s + 1 is multiplied from
-// LDS into the int32 accumulator of its order t = p + q (s accumulators of
-// 16 registers).  After each K chunk of OZ_KCHUNK the orders are folded
-// into fp64, smallest first (D = fma(D_t, 2^-7t, D) for t = s + 1 .. 2);
-// then the epilogue C = fma(D * 2^(e_i + f_j), alpha, fl(beta * C)).
+// Slice products with fp64 fold (general path): one BM x 64 tile of C at a
+// time per workgroup, persistent over the tiles; BM / 32 x 2 waves of
+// 32 x 32, each on v_mfma_i32_32x32x32_i8 (BM = 64: four waves, one per
+// SIMD; BM = 128: eight, two per SIMD -- fewer slice bytes per product).  A
+// step is one k-block of 32: every plane of the tile's A rows and B columns
+// moves global -> LDS with global_load_lds_dwordx4 into one of the stages.
+// The loads run ahead through the workgroup's whole sequence of tiles, so a
+// tile's first k-blocks arrive while the previous tile finishes; the tile's
+// row and column exponents ride along with its first step (into LDS, so
+// that the epilogue never waits on the vector memory counter behind later
+// loads).  Every slice pair (p, q) of the triangle p + q <= s + 1 is
+// multiplied from LDS into the int32 accumulator of its order t = p + q
+// (s accumulators of 16 registers).  After each K chunk of OZ_KCHUNK the
+// orders are folded into fp64, smallest first (D = fma(D_t, 2^-7t, D) for
+// t = s + 1 .. 2); then the epilogue C = fma(D * 2^(e_i + f_j), alpha,
+// fl(beta * C)).
 // Every step loads all S planes (an instance with fewer slices multiplies
 // only its own), so the steps are equal and the waits fixed counts.
 // LDS rows are 32 bytes (two 16-byte chunks); chunk c of row r sits at chunk
 // c ^ ((r >> 3) & 1), so a fragment read (rows r0 .. r0 + 15 of one chunk
 // column) covers 16 distinct bank groups.
 // ---------------------------------------------------------------------------
-template <int S>
+template <int S, int BM>
 struct OzMma {
-    static constexpr int PLANE = OZ_ROWS * OZ_KB;  // 2 KiB: one plane of the tile's rows
-    static constexpr int STAGE = 2 * S * PLANE;    // A planes, then B planes
-    static constexpr int NST = 4;                  // stages
-    static constexpr int AHEAD = NST - 1;          // steps loaded ahead
-    static constexpr int DPW = S;                  // loads per wave per step (4 S KiB over 4 waves)
-    static constexpr int EXP = NST * STAGE;        // exponents: 4 tiles x (64 rows + 64 columns) int32
-    static constexpr int LDS_BYTES = EXP + 4 * 2 * OZ_ROWS * 4;
+    static constexpr int WM = BM / 32, NW = 2 * WM;  // waves along M, waves
+    static constexpr int PLANE_A = BM * OZ_KB;       // one plane of the tile's A rows, bytes
+    static constexpr int PLANE_B = OZ_ROWS * OZ_KB;  // of its B columns
+    static constexpr int STAGE = S * (PLANE_A + PLANE_B);
+    static constexpr int NB = STAGE / 1024;          // 1 KiB load blocks per step
+    static constexpr int EXP_TILE = (BM + OZ_ROWS) * 4;
+    static constexpr int NST_FIT = (160 * 1024 - 4 * EXP_TILE) / STAGE;
+    static constexpr int NST = NST_FIT < 4 ? NST_FIT : 4;  // stages
+    static constexpr int AHEAD = NST - 1;                  // steps loaded ahead
+    static constexpr int DPW = NB / NW;                    // loads per wave per step (at least)
+    static constexpr int EXP = NST * STAGE;                // exponents: 4 tiles x (BM rows + 64 columns) int32
+    static constexpr int LDS_BYTES = EXP + 4 * EXP_TILE;
+    static_assert(NST >= 2, "at least two stages");
     static_assert(LDS_BYTES <= 160 * 1024, "stages fit the LDS");
 };
 
@@ -565,6 +573,7 @@ __device__ __forceinline__ int oz_chunk(int row, int c) { return row * OZ_KB + (
 struct OzTile {
     int inst, i0, j0;
 };
+template <int BM>
 __device__ __forceinline__ OzTile oz_tile(const OzParams& p, int i)
 {
     const uint32_t t = blockIdx.x + (uint32_t)i * gridDim.x;
@@ -574,7 +583,7 @@ __device__ __forceinline__ OzTile oz_tile(const OzParams& p, int i)
     const uint32_t tpi = (uint32_t)(p.tiles_m * p.tiles_n);
     const uint32_t inst = w / tpi, rem = w - inst * tpi;
     const uint32_t tm = rem % (uint32_t)p.tiles_m;
-    return OzTile{(int)inst, (int)tm * OZ_ROWS, (int)((rem - tm) / (uint32_t)p.tiles_m) * OZ_ROWS};
+    return OzTile{(int)inst, (int)tm * BM, (int)((rem - tm) / (uint32_t)p.tiles_m) * OZ_ROWS};
 }
 
 // A byte of a read-only device array through the scalar cache (a vector
@@ -597,10 +606,10 @@ struct OzStream {
     const char* b;  // and of its B columns
     int inst, i0, j0;
 };
-template <int S>
+template <int BM>
 __device__ __forceinline__ void oz_stream_tile(const OzParams& p, OzStream& L)
 {
-    const OzTile T = oz_tile(p, L.i);
+    const OzTile T = oz_tile<BM>(p, L.i);
     L.inst = T.inst;
     L.i0 = T.i0;
     L.j0 = T.j0;
@@ -609,30 +618,40 @@ __device__ __forceinline__ void oz_stream_tile(const OzParams& p, OzStream& L)
 }
 
 // Loads of step L.g into stage g % NST, then advance.  1 KiB block blk of
-// the stage (blk = wave + 4 j): plane blk / 2 (A planes first), rows
-// 32 (blk % 2) .. + 31; lane l fills row 32 (blk % 2) + l / 2, LDS chunk
-// l % 2, with the source chunk that belongs there.  A tile's first step
-// also brings its exponents (waves 0 and 1).
-template <int S>
+// the stage (blk = wave + NW j): A's planes first (BM / 32 blocks each),
+// then B's (two each); a block is 32 rows of one plane, lane l fills row
+// l / 2 of it, LDS chunk l % 2, with the source chunk that belongs there.
+// A tile's first step also brings its exponents (A rows: waves 0 ..
+// BM / 64 - 1, B columns: wave BM / 64).
+template <int S, int BM>
 __device__ __forceinline__ void oz_issue(const OzParams& p, OzStream& L, long gend, char* smem, int wave, int lane)
 {
-    using M = OzMma<S>;
+    using M = OzMma<S, BM>;
+    constexpr int AB = S * (BM / 32);  // A's blocks
     const long plane_a = p.rows_pad_a * OZ_KB, plane_b = p.rows_pad_b * OZ_KB;
-    char* st = smem + (int)(L.g & (M::NST - 1)) * M::STAGE;
-    const int src = oz_chunk(((wave & 1) << 5) + (lane >> 1), lane & 1);  // blk & 1 == wave & 1
+    char* st = smem + (int)(L.g % M::NST) * M::STAGE;
+    const int rl = lane >> 1, cl = lane & 1;
 #pragma unroll
-    for (int j = 0; j < S; ++j) {
-        const int blk = wave + 4 * j;  // 0 .. 4 S - 1
-        const int pl = blk >> 1;       // 0 .. 2 S - 1
-        const char* from = pl < S ? L.a + pl * plane_a + src : L.b + (pl - S) * plane_b + src;
-        __builtin_amdgcn_global_load_lds(from, (lds_void_t*)(st + blk * 1024), 16, 0, 0);
+    for (int j = 0; j < (M::NB + M::NW - 1) / M::NW; ++j) {
+        const int blk = wave + M::NW * j;
+        if (blk < M::NB) {
+            const char* from;
+            if (blk < AB) {
+                const int pl = blk / (BM / 32), row = (blk % (BM / 32)) * 32 + rl;
+                from = L.a + pl * plane_a + oz_chunk(row, cl);
+            } else {
+                const int b2 = blk - AB, pl = b2 >> 1, row = (b2 & 1) * 32 + rl;
+                from = L.b + pl * plane_b + oz_chunk(row, cl);
+            }
+            __builtin_amdgcn_global_load_lds(from, (lds_void_t*)(st + blk * 1024), 16, 0, 0);
+        }
     }
-    if (L.kb == 0 && wave < 2) {
-        // row (wave 0) or column (wave 1) exponents of the tile; lanes past
-        // the operand read its last one
-        const int32_t* e = wave == 0 ? p.ea + (long)L.inst * p.m + min(L.i0 + lane, p.m - 1)
-                                     : p.eb + (long)L.inst * p.n + min(L.j0 + lane, p.n - 1);
-        __builtin_amdgcn_global_load_lds(e, (lds_void_t*)(smem + M::EXP + ((L.i & 3) * 2 + wave) * OZ_ROWS * 4), 4,
+    if (L.kb == 0 && wave <= BM / 64) {
+        // row (waves 0 .. BM / 64 - 1) or column (wave BM / 64) exponents of
+        // the tile; lanes past the operand read its last one
+        const int32_t* e = wave < BM / 64 ? p.ea + (long)L.inst * p.m + min(L.i0 + wave * 64 + lane, p.m - 1)
+                                          : p.eb + (long)L.inst * p.n + min(L.j0 + lane, p.n - 1);
+        __builtin_amdgcn_global_load_lds(e, (lds_void_t*)(smem + M::EXP + (L.i & 3) * M::EXP_TILE + wave * 256), 4,
                                          0, 0);
     }
     ++L.g;
@@ -641,20 +660,22 @@ __device__ __forceinline__ void oz_issue(const OzParams& p, OzStream& L, long ge
     if (++L.kb == p.nkb) {
         L.kb = 0;
         ++L.i;
-        if (L.g < gend) oz_stream_tile<S>(p, L);
+        if (L.g < gend) oz_stream_tile<BM>(p, L);
     }
 }
 
 // One tile: steps g .. g + nkb - 1 of the workgroup's sequence (gend steps
 // in all).  FULL: the instance takes all S slices (no per-pair test).
-template <int S, bool FULL>
+// CHUNKS: K spans several chunks (the fp64 sums then stay live across the
+// k-loop; with one chunk they only exist after it).
+template <int S, int BM, bool FULL, bool CHUNKS>
 __device__ __forceinline__ void oz_mma_tile(const OzParams& p, const OzTile& T, int i, int si, char* smem, long& g,
                                             OzStream& L, long gend)
 {
-    using M = OzMma<S>;
+    using M = OzMma<S, BM>;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % M::WM, wn = wave / M::WM;
     const int fr_a = wm * 32 + (lane & 31), fr_b = wn * 32 + (lane & 31), kh = lane >> 5;
     const int nkb = p.nkb;
 
@@ -665,7 +686,7 @@ __device__ __forceinline__ void oz_mma_tile(const OzParams& p, const OzTile& T, 
     // k-block loop (a fold inside it would make the compiler move all of
     // them between accumulation and ordinary registers every k-block)
     for (int c0 = 0; c0 < nkb; c0 += OZ_KCHUNK / OZ_KB) {
-        const int c1 = min(nkb, c0 + OZ_KCHUNK / OZ_KB);
+        const int c1 = CHUNKS ? min(nkb, c0 + OZ_KCHUNK / OZ_KB) : nkb;
         i32x16 acc[S];
 #pragma unroll
         for (int t = 0; t < S; ++t) acc[t] = i32x16{};
@@ -676,15 +697,15 @@ __device__ __forceinline__ void oz_mma_tile(const OzParams& p, const OzTile& T, 
             if (g + M::AHEAD - 1 < gend) wait_vm<(M::AHEAD - 1) * M::DPW>();
             else wait_vm<0>();
             lds_barrier();
-            if (L.g < gend) oz_issue<S>(p, L, gend, smem, wave, lane);  // step g + AHEAD
+            if (L.g < gend) oz_issue<S, BM>(p, L, gend, smem, wave, lane);  // step g + AHEAD
             const char* la = smem + (int)(g % M::NST) * M::STAGE;
-            const char* lb = la + S * M::PLANE;
+            const char* lb = la + S * M::PLANE_A;
             u32x4 fa[S], fb[S];
 #pragma unroll
             for (int q = 0; q < S; ++q) {
                 if (FULL || q < si) {
-                    fa[q] = *reinterpret_cast<const u32x4*>(la + q * M::PLANE + oz_chunk(fr_a, kh));
-                    fb[q] = *reinterpret_cast<const u32x4*>(lb + q * M::PLANE + oz_chunk(fr_b, kh));
+                    fa[q] = *reinterpret_cast<const u32x4*>(la + q * M::PLANE_A + oz_chunk(fr_a, kh));
+                    fb[q] = *reinterpret_cast<const u32x4*>(lb + q * M::PLANE_B + oz_chunk(fr_b, kh));
                 }
             }
 #pragma unroll
@@ -702,14 +723,15 @@ __device__ __forceinline__ void oz_mma_tile(const OzParams& p, const OzTile& T, 
 #pragma unroll
             for (int v = 0; v < 16; ++v) D[v] = __builtin_fma((double)acc[t - 2][v], scale, D[v]);
         }
+        if (!CHUNKS) break;
     }
 
     // epilogue: lane -> column (lane & 31), accumulator v -> row
     // 8 (v / 4) + 4 (lane / 32) + v % 4 of the wave's 32 x 32 block
     const int c = T.j0 + wn * 32 + (lane & 31);
     if (c >= p.n) return;
-    const int32_t* ex = reinterpret_cast<const int32_t*>(smem + M::EXP + (int)(i & 3) * 2 * OZ_ROWS * 4);
-    const int fj = ex[OZ_ROWS + wn * 32 + (lane & 31)];
+    const int32_t* ex = reinterpret_cast<const int32_t*>(smem + M::EXP + (i & 3) * M::EXP_TILE);
+    const int fj = ex[BM + wn * 32 + (lane & 31)];
     double* C = wave_uniform(static_cast<double*>(p.C[T.inst]));
     const bool beta0 = p.beta0 != 0;
     const bool vec = (p.ldc % 2) == 0 && aligned16(C);
@@ -747,26 +769,26 @@ __device__ __forceinline__ void oz_mma_tile(const OzParams& p, const OzTile& T, 
     }
 }
 
-template <int S>
-__global__ void __launch_bounds__(256) ozaki_mma(const OzParams p)
+template <int S, int BM, bool CHUNKS>
+__global__ void __launch_bounds__(4 * BM) ozaki_mma(const OzParams p)  // 64 * OzMma<S, BM>::NW threads
 {
-    using M = OzMma<S>;
+    using M = OzMma<S, BM>;
     __shared__ __attribute__((aligned(1024))) char smem[M::LDS_BYTES];
     const int ntiles = p.total > (long)blockIdx.x ? (int)((p.total - 1 - blockIdx.x) / gridDim.x) + 1 : 0;
     const long gend = (long)ntiles * p.nkb;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     OzStream L{0, 0, 0, nullptr, nullptr, 0, 0, 0};
-    if (gend > 0) oz_stream_tile<S>(p, L);
+    if (gend > 0) oz_stream_tile<BM>(p, L);
 #pragma unroll
     for (int q = 0; q < M::AHEAD; ++q)
-        if (L.g < gend) oz_issue<S>(p, L, gend, smem, wave, lane);
+        if (L.g < gend) oz_issue<S, BM>(p, L, gend, smem, wave, lane);
     long g = 0;
     for (int i = 0; i < ntiles; ++i) {
-        const OzTile T = oz_tile(p, i);
+        const OzTile T = oz_tile<BM>(p, i);
         const int si = oz_scalar_byte(p.sarr, T.inst);
-        if (si >= S) oz_mma_tile<S, true>(p, T, i, S, smem, g, L, gend);
-        else oz_mma_tile<S, false>(p, T, i, si, smem, g, L, gend);
+        if (si >= S) oz_mma_tile<S, BM, true, CHUNKS>(p, T, i, S, smem, g, L, gend);
+        else oz_mma_tile<S, BM, false, CHUNKS>(p, T, i, si, smem, g, L, gend);
     }
 }
 
@@ -1182,14 +1204,15 @@ static hipError_t ozaki_small_dispatch(const OzParams& p, hipStream_t st)
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-template <int S>
-static hipError_t run_ozaki_mma_s(const OzParams& p0, hipStream_t st)
+template <int S, int BM, bool CHUNKS>
+static hipError_t run_ozaki_mma_k(const OzParams& p0, hipStream_t st)
 {
-    auto kern = ozaki_mma<S>;
-    const int occ = kernel_occupancy(kern, 256);
+    using M = OzMma<S, BM>;
+    auto kern = ozaki_mma<S, BM, CHUNKS>;
+    const int occ = kernel_occupancy(kern, 64 * M::NW);
     const long slots = (long)occ * device_cu_count();
     OzParams p = p0;
-    p.tiles_m = (p.m + OZ_ROWS - 1) / OZ_ROWS;
+    p.tiles_m = (p.m + BM - 1) / BM;
     p.tiles_n = (p.n + OZ_ROWS - 1) / OZ_ROWS;
     const long tpi = (long)p.tiles_m * p.tiles_n;
     // the kernel's tile arithmetic is 32-bit: launches of fewer than 2^31 tiles
@@ -1208,11 +1231,20 @@ static hipError_t run_ozaki_mma_s(const OzParams& p0, hipStream_t st)
         // a multiple of 8 workgroups keeps workgroup b on XCD b % 8 for every
         // tile it takes (xcd_remap)
         const long grid = p.total <= slots ? p.total : (slots / 8) * 8;
-        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, p);
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * M::NW), 0, st, p);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+template <int S>
+static hipError_t run_ozaki_mma_s(const OzParams& p, hipStream_t st)
+{
+    const bool chunks = p.nkb > OZ_KCHUNK / OZ_KB;
+    if (ozaki_tile_rows(p.m) == 128)
+        return chunks ? run_ozaki_mma_k<S, 128, true>(p, st) : run_ozaki_mma_k<S, 128, false>(p, st);
+    return chunks ? run_ozaki_mma_k<S, 64, true>(p, st) : run_ozaki_mma_k<S, 64, false>(p, st);
 }
 
 static hipError_t run_ozaki_mma(const OzParams& p, hipStream_t st)

@@ -69,7 +69,8 @@ OzakiLayout ozaki_layout(const GemmProblem& g, int s)
     OzakiLayout L{};
     const long m = g.m, n = g.n, k = g.k;
     const size_t batch = (size_t)g.batch;
-    L.rows_pad_a = (m + 63) / 64 * 64;
+    const long tr = ozaki_tile_rows(m);
+    L.rows_pad_a = (m + tr - 1) / tr * tr;
     L.rows_pad_b = (n + 63) / 64 * 64;
     L.nkb = (k + 63) / 64 * 2;  // k-blocks of 32, k padded to a multiple of 64
     L.inst_a = (size_t)L.nkb * s * L.rows_pad_a * 32;
