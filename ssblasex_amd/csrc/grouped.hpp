@@ -57,7 +57,7 @@ struct GItem {
 // Workgroup items are split into GQ queues, one per XCD: every tile of an
 // instance is in one queue, so the instance's operands are fetched into one
 // XCD's L2 (workgroups are dispatched to the eight XCDs round-robin, so
-// workgroup b runs on XCD b % 8 and takes from queue b % GQ first).
+// workgroup b runs on XCD b % 8 and takes its share of queue b % GQ).
 constexpr int GQ = 8;
 
 struct GroupedDevice {
@@ -70,11 +70,6 @@ struct GroupedDevice {
     const void* const* A;  // the caller's device arrays of device pointers
     const void* const* B;
     void* const* C;
-    // GQ + 2 counters, zero between launches: [q] next item of queue q,
-    // [GQ] workgroups done, [GQ + 1] next wave item.  The last workgroup of a
-    // launch resets them, so launches of one plan must be ordered (one
-    // stream, or a graph replayed on one stream).
-    unsigned* counters;
 };
 
 // k-depth of a grouped tile per form and shape.

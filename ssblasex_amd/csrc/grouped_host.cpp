@@ -14,7 +14,7 @@
 
 struct ssblasGroupedPlan {
     int device = 0;
-    void* mem = nullptr;  // counters, groups, instances, items: one device allocation
+    void* mem = nullptr;  // groups, instances, items: one device allocation
     int nfam = 0;
     int fam_id[ssb::GF_COUNT] = {};
     ssb::GroupedDevice fam[ssb::GF_COUNT] = {};
@@ -289,10 +289,9 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
     }
     if (nitems > INT_MAX) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
 
-    // Device image: [counters: 16 per family][groups][instances][items of each family]
-    static_assert(GQ + 2 <= 16 && 16 * GF_COUNT * sizeof(unsigned) <= 256, "counters fit the header");
+    // Device image: [groups][instances][items of each family]
     auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t off_groups = 256;
+    const size_t off_groups = 0;
     const size_t off_insts = align(off_groups + groups.size() * sizeof(GGroup));
     const size_t off_items = align(off_insts + insts.size() * sizeof(GInst));
     const size_t bytes = off_items + (size_t)std::max(nitems, 1L) * sizeof(GItem);
@@ -336,7 +335,6 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
         D.A = A;
         D.B = B;
         D.C = C;
-        D.counters = reinterpret_cast<unsigned*>(dev) + 16 * f;  // GQ + 2 <= 16 per family, 256-byte header
         p->fam_id[p->nfam] = f;
         ++p->nfam;
         pos += (nq + WL.size()) * sizeof(GItem);

@@ -4,11 +4,11 @@
 //
 // Work item = one output tile of one instance: the tile shape chosen for the
 // instance (32x32, 64x64 or 128x128), and through the instance its group's
-// arithmetic form, operand layouts and scalars.  Items are sorted by
-// estimated cost, longest first; workgroups take them from a device counter
-// one at a time, so long tiles start first and short ones fill in behind.
-// The last workgroup to finish puts the counters back to zero, which keeps a
-// launch free of memsets (a captured graph replays it as is).
+// arithmetic form, operand layouts and scalars.  Items sit in per-XCD
+// queues sorted by estimated cost, longest first, and every workgroup takes
+// a fixed share of its queue in turn with the others, so long tiles start
+// first and short ones fill in behind.  A launch writes nothing but C (a
+// captured graph replays it as is).
 //
 // Operands are read with buffer loads through a descriptor that spans the
 // stored matrix: whole 16-byte (or 4-byte) words at any leading dimension
@@ -372,12 +372,35 @@ __device__ __forceinline__ void g_run_wave(const TileJob& j)
     }
 }
 
+// The plan's tables and the caller's pointer arrays do not change during a
+// launch: read through the constant address space, at wave-uniform indices
+// they become scalar loads, which wait on their own counter instead of
+// behind the previous item's vector stores.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* g_const(const T* p)
+{
+    return (const __attribute__((address_space(4))) T*)(uintptr_t)p;
+}
+// Element idx of a table, word by word through the constant address space.
+template <class T>
+__device__ __forceinline__ T g_load(const T* base, long idx)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    const __attribute__((address_space(4))) uint32_t* w = g_const(reinterpret_cast<const uint32_t*>(base + idx));
+    uint32_t tmp[sizeof(T) / 4];
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) tmp[i] = w[i];
+    T out;
+    __builtin_memcpy(&out, tmp, sizeof(T));
+    return out;
+}
+
 __device__ __forceinline__ TileJob g_job(const GroupedDevice& d, const GItem& item, const GInst& in, const GGroup& g)
 {
     TileJob j;
-    j.A = static_cast<const char*>(d.A[item.inst]);
-    j.B = static_cast<const char*>(d.B[item.inst]);
-    j.C = d.C[item.inst];
+    j.A = static_cast<const char*>(g_const(d.A)[item.inst]);
+    j.B = static_cast<const char*>(g_const(d.B)[item.inst]);
+    j.C = g_const(d.C)[item.inst];
     j.lda = in.lda;
     j.ldb = in.ldb;
     j.ldc = in.ldc;
@@ -394,87 +417,46 @@ __device__ __forceinline__ TileJob g_job(const GroupedDevice& d, const GItem& it
 }
 
 // FA: the family's first form; FB: its second (== FA for one-form families).
-// Workgroup items first (the longer ones), from the workgroup's own XCD
-// queue and, once that is empty, from the other queues in turn; then wave
-// items: each wave of a workgroup takes W32 items from their own counter
-// until none is left.
+// Workgroup items first (the longer ones): workgroup b, on XCD b % 8, takes
+// the items of queue q = b % GQ in turn with the other workgroups of that
+// queue (rank r = b / GQ of nq: items r, r + nq, r + 2 nq, ... -- the queue
+// is sorted longest first, so every workgroup gets one item of each stretch
+// of similar cost).  Then wave items, the same way over all the grid's
+// waves.  The assignment is fixed: no claims, so nothing waits on an atomic
+// (whose return, on gfx9, retires in order behind the previous tile's
+// stores) or on the barriers that shared a claimed number.
 template <class FA, class FB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) grouped_gemm(const GroupedDevice d)
 {
     constexpr int LA = g_lds_max<FA>(), LB = g_lds_max<FB>();
     __shared__ __attribute__((aligned(16))) char smem[LA > LB ? LA : LB];
-    __shared__ int s_item;
     const int tid = threadIdx.x;
-    int q = blockIdx.x & (GQ - 1), visited = 0;  // this workgroup's XCD queue first
-    // The next item of queue q, moving on to the next queue while q is
-    // empty; -1 once every queue is.  Wave-uniform (read back through LDS).
-    auto claim = [&]() -> int {
-        while (visited < GQ) {
-            if (tid == 0) s_item = (int)atomicAdd(&d.counters[q], 1u);
-            __syncthreads();
-            const int v = __builtin_amdgcn_readfirstlane(s_item);
-            __syncthreads();
-            if (v < d.qoff[q + 1] - d.qoff[q]) return d.qoff[q] + v;
-            q = (q + 1) & (GQ - 1);
-            ++visited;
-        }
-        return -1;
-    };
-    // workgroups that start from one queue
-    const int per_queue = ((int)gridDim.x + GQ - 1) / GQ;
-    int it = claim();
-    while (it >= 0) {
-        // Claim the queue's next item now (its number is read once this one
-        // is done, so the atomic's round trip overlaps this item's work) --
-        // except in the last two rounds of the queue, where holding a second
-        // item would leave other workgroups idle at the end.
-        const int qend = d.qoff[q + 1];
-        const bool early = it + 2 * per_queue < qend;
-        unsigned next = 0;
-        if (tid == 0 && early) next = atomicAdd(&d.counters[q], 1u);
-        const GItem item = d.items[it];
-        const GInst in = d.insts[item.inst];
-        const GGroup g = d.groups[in.group];
-        const TileJob j = g_job(d, item, in, g);
-        if (g.form == FA::ID) g_run<FA>(item.cfg, j, in.abytes, in.bbytes, smem);
-        else g_run<FB>(item.cfg, j, in.abytes, in.bbytes, smem);
-        if (early) {
-            if (tid == 0) s_item = (int)next;
-            __syncthreads();
-            const int v = __builtin_amdgcn_readfirstlane(s_item);
-            __syncthreads();
-            it = d.qoff[q] + v < qend ? d.qoff[q] + v : claim();
-        } else {
-            it = claim();
+    // with fewer workgroups than queues, workgroup b also takes queues
+    // b + G, b + 2 G, ... (G = gridDim.x) on its own
+    const int G = (int)gridDim.x;
+    const int qstep = G < GQ ? G : GQ;
+    for (int q = (int)blockIdx.x % GQ; q < GQ; q += qstep) {
+        const int nq = G < GQ ? 1 : (G - q + GQ - 1) / GQ;  // workgroups taking queue q
+        const int rank = G < GQ ? 0 : (int)blockIdx.x / GQ;
+        for (int it = d.qoff[q] + rank; it < d.qoff[q + 1]; it += nq) {
+            const GItem item = g_load(d.items, it);
+            const GInst in = g_load(d.insts, item.inst);
+            const GGroup g = g_load(d.groups, in.group);
+            const TileJob j = g_job(d, item, in, g);
+            if (g.form == FA::ID) g_run<FA>(item.cfg, j, in.abytes, in.bbytes, smem);
+            else g_run<FB>(item.cfg, j, in.abytes, in.bbytes, smem);
         }
     }
     if (d.nwitems > 0) {
-        const int lane = tid & 63;
-        auto claim_wave = [&]() {
-            unsigned v = 0u;
-            if (lane == 0) v = atomicAdd(&d.counters[GQ + 1], 1u);
-            return __builtin_amdgcn_readfirstlane((int)v);  // lane 0's claim, for the whole wave
-        };
-        for (int w = claim_wave(); w < d.nwitems; w = claim_wave()) {
-            const GItem item = d.items[d.nitems + w];
-            const GInst in = d.insts[item.inst];
-            const GGroup g = d.groups[in.group];
+        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        const int nw = 4 * (int)gridDim.x;
+        for (int w = 4 * (int)blockIdx.x + wave; w < d.nwitems; w += nw) {
+            const GItem item = g_load(d.items, d.nitems + w);
+            const GInst in = g_load(d.insts, item.inst);
+            const GGroup g = g_load(d.groups, in.group);
             const TileJob j = g_job(d, item, in, g);
             if (g.form == FA::ID) g_run_wave<FA>(j);
             else g_run_wave<FB>(j);
-        }
-    }
-    __syncthreads();  // every wave of the workgroup is done with its items
-    // Every workgroup gets here once; the last one re-arms the counters for
-    // the next launch (all other workgroups made their final counter updates
-    // before their fenced arrival).
-    if (tid == 0) {
-        __threadfence();
-        const unsigned arrived = atomicAdd(&d.counters[GQ], 1u);
-        if (arrived == gridDim.x - 1) {
-            __threadfence();
-#pragma unroll
-            for (int c = 0; c < GQ + 2; ++c) atomicExch(&d.counters[c], 0u);
         }
     }
 }

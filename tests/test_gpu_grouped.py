@@ -123,8 +123,8 @@ def test_grouped_fp64_and_bf16x3_families(cfg, monkeypatch):
 @pytest.mark.parametrize("cfg", [None, "w32:g64"])
 def test_grouped_graph_replays(cfg, monkeypatch):
     """Captured graph replayed with eager work between replays: each replay
-    matches the oracle on the data it saw (the work counters, wave items'
-    included, are back at zero after every launch)."""
+    matches the oracle on the data it saw (a launch keeps no state between
+    replays)."""
     import torch
 
     if cfg is not None:
