@@ -27,32 +27,29 @@ enum GroupedFamily : int {
     GF_COUNT = 4
 };
 
-struct GScalars {  // same layout as the device Scalars
-    float af, bf;
-    double ad, bd;
-    int32_t ai, bi;
-};
-
-struct GGroup {
-    int form;   // ArithKind
-    int akc;    // op(A) stored K-contiguous (A transposed)
-    int bkc;    // op(B) stored K-contiguous (B not transposed)
-    int beta0;
-    GScalars s;
-};
-
-struct GInst {
-    int m, n, k, group;
-    long lda, ldb, ldc;
+// One work item: one output tile of one instance, as a 64-byte record that
+// holds everything the kernel needs except the operand pointers (those are
+// read from the caller's arrays when the item runs, so a plan follows
+// changes of the pointers).  One record is one 16-lane vector load.
+struct GRec {
+    int32_t inst;      // instance index (into the caller's A / B / C arrays)
+    int32_t i0, j0;    // tile origin in C
+    uint32_t bits;     // cfg | form << 2 | akc << 5 | bkc << 6 | beta0 << 7
+    int32_t m, n, k;
+    int32_t lda, ldb, ldc;
     uint32_t abytes, bbytes;  // extent of the stored A / B in bytes
+    uint32_t alpha[2];  // the group's scalars in their scale type's bits
+    uint32_t beta[2];   // (fp32 and int32 in word 0, fp64 in both words)
 };
+static_assert(sizeof(GRec) == 64, "one record = 16 dwords");
+constexpr int GREC_WORDS = 16;
 
-// One output tile of one instance.
-struct GItem {
-    int inst;
-    int i0, j0;
-    int cfg;    // GroupedCfg
-};
+// Fields of GRec::bits.
+constexpr uint32_t grec_bits(int cfg, int form, int akc, int bkc, int beta0)
+{
+    return (uint32_t)cfg | ((uint32_t)form << 2) | ((uint32_t)akc << 5) | ((uint32_t)bkc << 6) |
+           ((uint32_t)beta0 << 7);
+}
 
 // Workgroup items are split into GQ queues, one per XCD: every tile of an
 // instance is in one queue, so the instance's operands are fetched into one
@@ -61,12 +58,10 @@ struct GItem {
 constexpr int GQ = 8;
 
 struct GroupedDevice {
-    const GItem* items;  // workgroup items queue by queue, then wave items; each sorted by estimated cost, longest first
+    const GRec* recs;    // workgroup items queue by queue, then wave items; each sorted by estimated cost, longest first
     int nitems;          // workgroup items (all queues)
-    int nwitems;         // wave items (items[nitems .. nitems + nwitems))
-    int qoff[GQ + 1];    // queue q holds items[qoff[q] .. qoff[q + 1])
-    const GInst* insts;
-    const GGroup* groups;
+    int nwitems;         // wave items (recs[nitems .. nitems + nwitems))
+    int qoff[GQ + 1];    // queue q holds recs[qoff[q] .. qoff[q + 1])
     const void* const* A;  // the caller's device arrays of device pointers
     const void* const* B;
     void* const* C;

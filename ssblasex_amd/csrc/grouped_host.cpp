@@ -137,10 +137,45 @@ uint32_t extent_bytes(long rows, long cols, long ld, int e)
     return (uint32_t)ext;
 }
 
+// What the plan keeps per group and per instance while it builds the records.
+struct HostGroup {
+    int form;   // ArithKind
+    int akc;    // op(A) stored K-contiguous (A transposed)
+    int bkc;    // op(B) stored K-contiguous (B not transposed)
+    int beta0;
+    uint32_t alpha[2], beta[2];  // scalar bits as in GRec
+};
+struct HostInst {
+    int m, n, k, group;
+    long lda, ldb, ldc;
+    uint32_t abytes, bbytes;
+};
+
 struct HostItem {
-    GItem it;
+    GRec rec;
     double ns;
 };
+
+// The group's alpha / beta in the bits the kernel reads for its form.
+void scalar_bits(int form, const GemmProblem& sc, uint32_t (&a)[2], uint32_t (&b)[2])
+{
+    a[1] = b[1] = 0u;
+    if (form == K_I8) {
+        a[0] = (uint32_t)sc.ai;
+        b[0] = (uint32_t)sc.bi;
+    } else if (form == K_F64) {
+        uint64_t x, y;
+        memcpy(&x, &sc.ad, 8);
+        memcpy(&y, &sc.bd, 8);
+        a[0] = (uint32_t)x;
+        a[1] = (uint32_t)(x >> 32);
+        b[0] = (uint32_t)y;
+        b[1] = (uint32_t)(y >> 32);
+    } else {
+        memcpy(&a[0], &sc.af, 4);
+        memcpy(&b[0], &sc.bf, 4);
+    }
+}
 
 }  // namespace
 
@@ -184,7 +219,7 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
     if (total == 0) return SSBLAS_STATUS_INTERNAL_ERROR;
     if (!m_array || !n_array || !k_array || !lda_array || !ldb_array || !ldc_array) return SSBLAS_STATUS_INVALID_VALUE;
 
-    std::vector<GGroup> groups((size_t)group_count);
+    std::vector<HostGroup> groups((size_t)group_count);
     for (long g = 0; g < group_count; ++g) {
         TypeTriple tt = classify_types(Atype_array[g], Btype_array[g], Ctype_array[g]);
         if (tt == TT_NONE) return SSBLAS_STATUS_INVALID_VALUE;
@@ -194,15 +229,15 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
         if (alpha_array[g] == nullptr || beta_array[g] == nullptr) return SSBLAS_STATUS_INVALID_VALUE;
         GemmProblem sc;
         fill_scalars(tt, alpha_array[g], beta_array[g], sc);
-        GGroup& G = groups[(size_t)g];
+        HostGroup& G = groups[(size_t)g];
         G.form = tt == TT_I8 ? K_I8 : tt == TT_F64 ? K_F64 : fp32_form;
         G.akc = ta == SSBLAS_OP_T ? 1 : 0;
         G.bkc = tb == SSBLAS_OP_N ? 1 : 0;
         G.beta0 = sc.beta0 ? 1 : 0;
-        G.s = GScalars{sc.af, sc.bf, sc.ad, sc.bd, sc.ai, sc.bi};
+        scalar_bits(G.form, sc, G.alpha, G.beta);
     }
 
-    std::vector<GInst> insts((size_t)total);
+    std::vector<HostInst> insts((size_t)total);
     long idx = 0;
     for (long g = 0; g < group_count; ++g) {
         const ssblasOperation_t ta = transa_array[g], tb = transb_array[g];
@@ -216,8 +251,8 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
             const long rb = stored_rows(tb, k, n), cb = stored_rows(tb, n, k);
             if (lda < std::max(ra, 1L) || ldb < std::max(rb, 1L) || ldc < std::max(m, 1L))
                 return SSBLAS_STATUS_INVALID_VALUE;
-            if (m > INT_MAX || n > INT_MAX || k > INT_MAX) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
-            GInst& I = insts[(size_t)idx];
+            if (m > INT_MAX || n > INT_MAX || k > INT_MAX || ldc > INT_MAX) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
+            HostInst& I = insts[(size_t)idx];
             I.m = (int)m;
             I.n = (int)n;
             I.k = (int)k;
@@ -248,9 +283,9 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
     std::vector<HostItem> tiles;
     std::vector<InstWork> work;
     for (long i = 0; i < total; ++i) {
-        const GInst& I = insts[(size_t)i];
+        const HostInst& I = insts[(size_t)i];
         if (I.m == 0 || I.n == 0 || I.k == 0) continue;
-        const GGroup& G = groups[(size_t)I.group];
+        const HostGroup& G = groups[(size_t)I.group];
         const int cfg = pick_cfg(G.form, I.m, I.n, I.k, G.beta0 != 0);
         const int t = tile_of(cfg);
         const double ns = item_ns(G.form, cfg, I.k, G.beta0 != 0);
@@ -258,7 +293,22 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
         const size_t begin = tiles.size();
         for (int j0 = 0; j0 < I.n; j0 += t)
             for (int i0 = 0; i0 < I.m; i0 += t) {
-                const HostItem h{GItem{(int)i, i0, j0, cfg}, ns};
+                GRec r;
+                r.inst = (int32_t)i;
+                r.i0 = i0;
+                r.j0 = j0;
+                r.bits = grec_bits(cfg, G.form, G.akc, G.bkc, G.beta0);
+                r.m = I.m;
+                r.n = I.n;
+                r.k = I.k;
+                r.lda = (int32_t)I.lda;  // below 2^31: the extent fits 31 bits
+                r.ldb = (int32_t)I.ldb;
+                r.ldc = (int32_t)I.ldc;
+                r.abytes = I.abytes;
+                r.bbytes = I.bbytes;
+                memcpy(r.alpha, G.alpha, sizeof(r.alpha));
+                memcpy(r.beta, G.beta, sizeof(r.beta));
+                const HostItem h{r, ns};
                 if (cfg == GCFG_W32) wlists[f].push_back(h);
                 else tiles.push_back(h);
             }
@@ -289,19 +339,14 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
     }
     if (nitems > INT_MAX) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
 
-    // Device image: [groups][instances][items of each family]
-    auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-    const size_t off_groups = 0;
-    const size_t off_insts = align(off_groups + groups.size() * sizeof(GGroup));
-    const size_t off_items = align(off_insts + insts.size() * sizeof(GInst));
-    const size_t bytes = off_items + (size_t)std::max(nitems, 1L) * sizeof(GItem);
+    // Device image: the records of each family
+    const size_t off_items = 0;
+    const size_t bytes = (size_t)std::max(nitems, 1L) * sizeof(GRec);
     // the plan's device image is library-owned workspace
     if (bytes > workspace_limit()) return SSBLAS_STATUS_ALLOC_FAILED;
     if (!device_accessible(A) || !device_accessible(B) || !device_accessible(C))
         return SSBLAS_STATUS_INVALID_VALUE;
     std::vector<char> image(bytes, 0);
-    memcpy(image.data() + off_groups, groups.data(), groups.size() * sizeof(GGroup));
-    memcpy(image.data() + off_insts, insts.data(), insts.size() * sizeof(GInst));
 
     ssblasGroupedPlan* p = new (std::nothrow) ssblasGroupedPlan();
     if (p == nullptr) return SSBLAS_STATUS_ALLOC_FAILED;
@@ -318,26 +363,24 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
         for (int qi = 0; qi < GQ; ++qi) nq += qlists[f][qi].size();
         auto& WL = wlists[f];
         if (nq == 0 && WL.empty()) continue;
-        GItem* dst = reinterpret_cast<GItem*>(image.data() + pos);
+        GRec* dst = reinterpret_cast<GRec*>(image.data() + pos);
         GroupedDevice& D = p->fam[p->nfam];
         size_t x = 0;
         for (int qi = 0; qi < GQ; ++qi) {
             D.qoff[qi] = (int)x;
-            for (const HostItem& h : qlists[f][qi]) dst[x++] = h.it;
+            for (const HostItem& h : qlists[f][qi]) dst[x++] = h.rec;
         }
         D.qoff[GQ] = (int)x;
-        for (size_t i = 0; i < WL.size(); ++i) dst[x + i] = WL[i].it;
-        D.items = reinterpret_cast<const GItem*>(dev + pos);
+        for (size_t i = 0; i < WL.size(); ++i) dst[x + i] = WL[i].rec;
+        D.recs = reinterpret_cast<const GRec*>(dev + pos);
         D.nitems = (int)nq;
         D.nwitems = (int)WL.size();
-        D.insts = reinterpret_cast<const GInst*>(dev + off_insts);
-        D.groups = reinterpret_cast<const GGroup*>(dev + off_groups);
         D.A = A;
         D.B = B;
         D.C = C;
         p->fam_id[p->nfam] = f;
         ++p->nfam;
-        pos += (nq + WL.size()) * sizeof(GItem);
+        pos += (nq + WL.size()) * sizeof(GRec);
     }
     e = hipMemcpy(p->mem, image.data(), bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {

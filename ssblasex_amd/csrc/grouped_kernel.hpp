@@ -13,9 +13,9 @@
 // Operands are read with buffer loads through a descriptor that spans the
 // stored matrix: whole 16-byte (or 4-byte) words at any leading dimension
 // and alignment, where a dword past the end of the matrix reads as zero
-// instead of faulting.  Elements outside the tile's rows or k range are then
-// cleared in registers before the LDS store, so partial tiles take the same
-// path as whole ones.
+// instead of faulting.  Partial tiles take the same loads as whole ones;
+// only a k tail (and, for int8, the word holding a matrix's last bytes) is
+// fixed in registers before the LDS store.
 //
 // Included by one .hip file per family.
 #pragma once
@@ -44,12 +44,21 @@ __device__ __forceinline__ void clear_tail(uint32_t* d, int nvalid)
 // Buffer-load staging of one operand tile (ROWS x BK), same unit and word
 // layout as StagerImpl<KC>, so StagerImpl::store writes it to LDS.
 //
-// Whole tiles read plain words (every word lies inside the matrix).  In a
-// partial tile a word may cross the end of the stored matrix (`ext` bytes);
-// the descriptor's range check drops a dword that reaches past the end, so
-// such a word is read instead as the WB bytes that end exactly at `ext` and
-// shifted into place by `fix`.  A matrix smaller than one word is read
-// element by element.
+// Every word is read whole through the descriptor, which spans the stored
+// matrix (`ext` bytes): a dword past the end reads as zero.  Rows (columns)
+// outside the instance are not masked: they only feed rows (columns) of the
+// product that are never stored.  What has to be fixed:
+//  * a K-contiguous operand in a k-block that reaches past K: a word's
+//    elements at k >= K hold padding or the next row and are cleared
+//    (`clear_k`); an M/N-contiguous operand's words at k >= K start past the
+//    end of the matrix and read as zero;
+//  * int8: the dword holding the matrix's last bytes can reach past the end
+//    and read as zero as well, so a word that crosses the end is read again
+//    as the WB bytes ending exactly at `ext` and shifted into place
+//    (`fix_end`; a wave takes that branch only when one of its lanes holds
+//    such a word, which only the tile holding the last element of the
+//    matrix, in its last k-block, can have).  A matrix smaller than one word
+//    is read element by element there.
 template <class AR, bool KC, int ROWS, int BK, int NT>
 struct GLoad {
     using I = StagerImpl<AR, KC, ROWS, BK, NT>;
@@ -97,8 +106,8 @@ struct GLoad {
     }
 
     // off0: byte offset of the tile origin (row 0, k 0 of the k-block).
-    __device__ __forceinline__ static void load(uint32_t* raw, __amdgpu_buffer_rsrc_t rs, uint32_t off0, uint32_t ext,
-                                                long sr, long sk, bool partial, int tid)
+    __device__ __forceinline__ static void load(uint32_t* raw, __amdgpu_buffer_rsrc_t rs, uint32_t off0, long sr,
+                                                long sk, int tid)
     {
 #pragma unroll
         for (int x = 0; x < I::UPT; ++x) {
@@ -107,54 +116,73 @@ struct GLoad {
             int r0, k0;
             I::unit_coords(u, r0, k0);
 #pragma unroll
+            for (int w = 0; w < I::NW; ++w)
+                read_word(raw + x * I::UD + w * I::WD, rs, off0 + (uint32_t)I::word_off(r0, k0, w, sr, sk));
+        }
+    }
+
+    // Words that cross the end of the matrix, read again and shifted.
+    __device__ __forceinline__ static void fix_end(uint32_t* raw, __amdgpu_buffer_rsrc_t rs, uint32_t off0,
+                                                   uint32_t ext, long sr, long sk, int tid)
+    {
+        bool any = false;
+#pragma unroll
+        for (int x = 0; x < I::UPT; ++x) {
+            const int u = tid + x * NT;
+            if (!I::active(u)) break;
+            int r0, k0;
+            I::unit_coords(u, r0, k0);
+#pragma unroll
+            for (int w = 0; w < I::NW; ++w) any |= crossing(off0 + (uint32_t)I::word_off(r0, k0, w, sr, sk), ext);
+        }
+        if (!__any(any)) return;
+#pragma unroll
+        for (int x = 0; x < I::UPT; ++x) {
+            const int u = tid + x * NT;
+            if (!I::active(u)) break;
+            int r0, k0;
+            I::unit_coords(u, r0, k0);
+#pragma unroll
             for (int w = 0; w < I::NW; ++w) {
-                const uint32_t off = off0 + (uint32_t)I::word_off(r0, k0, w, sr, sk);
                 uint32_t* d = raw + x * I::UD + w * I::WD;
-                if (!partial || !crossing(off, ext)) read_word(d, rs, off);
-                else if (ext >= (uint32_t)WB) read_word(d, rs, ext - WB);
-                else read_elems(d, rs, off, ext);
+                const uint32_t off = off0 + (uint32_t)I::word_off(r0, k0, w, sr, sk);
+                if (!crossing(off, ext)) continue;
+                if (ext < (uint32_t)WB) {
+                    read_elems(d, rs, off, ext);
+                    continue;
+                }
+                read_word(d, rs, ext - WB);
+                // byte j of the word is byte j + sh of what was read
+                const uint32_t sh = off - (ext - WB);
+                if constexpr (WB == 16) {
+                    const uint32_t q = sh >> 2, r = sh & 3;
+                    uint32_t t[5];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t a = i + q;
+                        t[i] = a == 0 ? d[0] : a == 1 ? d[1] : a == 2 ? d[2] : a == 3 ? d[3] : 0u;
+                    }
+                    t[4] = 0u;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) d[i] = __builtin_amdgcn_alignbyte(t[i + 1], t[i], r);
+                } else {
+                    d[0] = __builtin_amdgcn_alignbyte(0u, d[0], sh);
+                }
             }
         }
     }
 
-    // Partial tiles: shift re-read words into place, then clear what lies
-    // outside rows_valid x k_valid (tile-relative).
-    __device__ __forceinline__ static void fix(uint32_t* raw, uint32_t off0, uint32_t ext, long sr, long sk,
-                                               int rows_valid, int k_valid, int tid)
+    // K-contiguous operand: clear the elements at tile-relative k >= k_valid.
+    __device__ __forceinline__ static void clear_k(uint32_t* raw, int k_valid, int tid)
     {
+        static_assert(KC, "M/N-contiguous words past K read as zero");
 #pragma unroll
         for (int x = 0; x < I::UPT; ++x) {
             const int u = tid + x * NT;
             if (!I::active(u)) break;
             int r0, k0;
             I::unit_coords(u, r0, k0);
-#pragma unroll
-            for (int w = 0; w < I::NW; ++w) {
-                uint32_t* d = raw + x * I::UD + w * I::WD;
-                const uint32_t off = off0 + (uint32_t)I::word_off(r0, k0, w, sr, sk);
-                if (crossing(off, ext) && ext >= (uint32_t)WB) {
-                    // byte j of the word is byte j + sh of what was read
-                    const uint32_t sh = off - (ext - WB);
-                    if constexpr (WB == 16) {
-                        const uint32_t q = sh >> 2, r = sh & 3;
-                        uint32_t t[5];
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const uint32_t a = i + q;
-                            t[i] = a == 0 ? d[0] : a == 1 ? d[1] : a == 2 ? d[2] : a == 3 ? d[3] : 0u;
-                        }
-                        t[4] = 0u;
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) d[i] = __builtin_amdgcn_alignbyte(t[i + 1], t[i], r);
-                    } else {
-                        d[0] = __builtin_amdgcn_alignbyte(0u, d[0], sh);
-                    }
-                }
-                int nv;
-                if constexpr (KC) nv = r0 < rows_valid ? min(max(k_valid - k0, 0), (int)I::WE) : 0;
-                else nv = k0 + w < k_valid ? min(max(rows_valid - r0, 0), (int)I::WE) : 0;
-                clear_tail<WB, E>(d, nv);
-            }
+            clear_tail<WB, E>(raw + x * I::UD, min(max(k_valid - k0, 0), (int)I::WE));
         }
     }
 };
@@ -167,17 +195,20 @@ struct GStage {
     static constexpr int RAWD = LK::I::RAWD > LN::I::RAWD ? LK::I::RAWD : LN::I::RAWD;
     uint32_t raw[RAWD];
 
-    __device__ __forceinline__ void load(bool kc, __amdgpu_buffer_rsrc_t rs, uint32_t off0, uint32_t ext, long sr,
-                                         long sk, bool partial, int tid)
+    __device__ __forceinline__ void load(bool kc, __amdgpu_buffer_rsrc_t rs, uint32_t off0, long sr, long sk, int tid)
     {
-        if (kc) LK::load(raw, rs, off0, ext, sr, sk, partial, tid);
-        else LN::load(raw, rs, off0, ext, sr, sk, partial, tid);
+        if (kc) LK::load(raw, rs, off0, sr, sk, tid);
+        else LN::load(raw, rs, off0, sr, sk, tid);
     }
-    __device__ __forceinline__ void fix(bool kc, uint32_t off0, uint32_t ext, long sr, long sk, int rows_valid,
-                                        int k_valid, int tid)
+    __device__ __forceinline__ void fix_end(bool kc, __amdgpu_buffer_rsrc_t rs, uint32_t off0, uint32_t ext, long sr,
+                                            long sk, int tid)
     {
-        if (kc) LK::fix(raw, off0, ext, sr, sk, rows_valid, k_valid, tid);
-        else LN::fix(raw, off0, ext, sr, sk, rows_valid, k_valid, tid);
+        if (kc) LK::fix_end(raw, rs, off0, ext, sr, sk, tid);
+        else LN::fix_end(raw, rs, off0, ext, sr, sk, tid);
+    }
+    __device__ __forceinline__ void clear_k(bool kc, int k_valid, int tid)
+    {
+        if (kc) LK::clear_k(raw, k_valid, tid);
     }
     __device__ __forceinline__ void store(bool kc, char* lds, int plane_bytes, int rowb, int tid) const
     {
@@ -238,8 +269,9 @@ __device__ __forceinline__ void gstore_frag(typename AR::Out* C, long ldc, int r
 // One BM x BN tile of one grouped instance, 4 waves (2 x 2), LDS
 // double-buffered, global loads of k-block kt+1 in flight while kt is
 // multiplied.  abytes / bbytes: extent of the stored A / B in bytes.
-template <class AR, int BM, int BN, int BK>
-__device__ __forceinline__ void gtile_job(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem)
+template <class AR, int BM, int BN, int BK, class Hook>
+__device__ __forceinline__ void gtile_job(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
+                                          Hook&& first_block)
 {
     using S = TileShape<AR, true, true, BM, BN, BK, 2, 2>;
     using Out = typename AR::Out;
@@ -260,8 +292,6 @@ __device__ __forceinline__ void gtile_job(const TileJob& j, uint32_t abytes, uin
     const long sra = AKC ? j.lda : 1, ska = AKC ? 1 : j.lda;
     const long srb = BKC ? j.ldb : 1, skb = BKC ? 1 : j.ldb;
     const int nk = (j.k + BK - 1) / BK;
-    const int rows_a = min(BM, j.m - j.i0);
-    const int rows_b = min(BN, j.n - j.j0);
     const __amdgpu_buffer_rsrc_t ra = make_rsrc(j.A, abytes);
     const __amdgpu_buffer_rsrc_t rb = make_rsrc(j.B, bbytes);
     const uint32_t a0 = (uint32_t)((long)j.i0 * sra * E), b0 = (uint32_t)((long)j.j0 * srb * E);
@@ -275,16 +305,28 @@ __device__ __forceinline__ void gtile_job(const TileJob& j, uint32_t abytes, uin
 #pragma unroll
         for (int fn = 0; fn < S::FN; ++fn) acc[fm][fn] = typename AR::Acc{};
 
-    // a k-block of a tile with rows or k outside the matrix is partial
     auto load_step = [&](int kt) {
-        const bool kpart = j.k - kt * BK < BK;
-        sa.load(AKC, ra, a0 + (uint32_t)kt * a_kstep, abytes, sra, ska, kpart || rows_a < BM, tid);
-        sb.load(BKC, rb, b0 + (uint32_t)kt * b_kstep, bbytes, srb, skb, kpart || rows_b < BN, tid);
+        sa.load(AKC, ra, a0 + (uint32_t)kt * a_kstep, sra, ska, tid);
+        sb.load(BKC, rb, b0 + (uint32_t)kt * b_kstep, srb, skb, tid);
     };
+    // int8 words crossing the end of a matrix: only where the k-block's
+    // footprint (first to last element it reads) reaches past the end; then
+    // the k tail of the last k-block
+    const uint32_t a_span = (uint32_t)(((BM - 1) * sra + (BK - 1) * ska + 1) * E);
+    const uint32_t b_span = (uint32_t)(((BN - 1) * srb + (BK - 1) * skb + 1) * E);
     auto store_step = [&](int kt, int buf) {
-        const int kv = j.k - kt * BK;
-        if (rows_a < BM || kv < BK) sa.fix(AKC, a0 + (uint32_t)kt * a_kstep, abytes, sra, ska, rows_a, kv, tid);
-        if (rows_b < BN || kv < BK) sb.fix(BKC, b0 + (uint32_t)kt * b_kstep, bbytes, srb, skb, rows_b, kv, tid);
+        if constexpr (E == 1) {
+            const uint32_t ao = a0 + (uint32_t)kt * a_kstep, bo = b0 + (uint32_t)kt * b_kstep;
+            if (ao + a_span > abytes) sa.fix_end(AKC, ra, ao, abytes, sra, ska, tid);
+            if (bo + b_span > bbytes) sb.fix_end(BKC, rb, bo, bbytes, srb, skb, tid);
+        }
+        if (kt == nk - 1) {
+            const int kv = j.k - kt * BK;
+            if (kv < BK) {
+                sa.clear_k(AKC, kv, tid);
+                sb.clear_k(BKC, kv, tid);
+            }
+        }
         char* base = smem + buf * S::BUF;
         sa.store(AKC, base, S::A_PLANE, S::ROWB, tid);
         sb.store(BKC, base + AR::NPLANE * S::A_PLANE, S::B_PLANE, S::ROWB, tid);
@@ -294,6 +336,7 @@ __device__ __forceinline__ void gtile_job(const TileJob& j, uint32_t abytes, uin
     __syncthreads();  // previous user of smem is done
     store_step(0, 0);
     __syncthreads();
+    first_block();
     for (int kt = 0; kt < nk; ++kt) {
         const int buf = kt & 1;
         if (kt + 1 < nk) load_step(kt + 1);
@@ -339,23 +382,24 @@ constexpr int g_lds_max()
     return grouped_max_cfg(AR::ID) == GCFG_128 ? (ab > c ? ab : c) : ab;
 }
 
-template <class AR>
-__device__ __forceinline__ void g_run(int cfg, const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem)
+template <class AR, class Hook>
+__device__ __forceinline__ void g_run(int cfg, const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
+                                      Hook&& first_block)
 {
     if (cfg == GCFG_32) {
         using G = GShape<AR, GCFG_32>;
-        gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem);
+        gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, first_block);
         return;
     }
     if constexpr (grouped_max_cfg(AR::ID) == GCFG_128) {
         if (cfg == GCFG_128) {
             using G = GShape<AR, GCFG_128>;
-            gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem);
+            gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, first_block);
             return;
         }
     }
     using G = GShape<AR, GCFG_64>;
-    gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem);
+    gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, first_block);
 }
 
 // One wave's 32x32 block of a grouped instance (W32 items), operand layouts
@@ -372,91 +416,161 @@ __device__ __forceinline__ void g_run_wave(const TileJob& j)
     }
 }
 
-// The plan's tables and the caller's pointer arrays do not change during a
-// launch: read through the constant address space, at wave-uniform indices
-// they become scalar loads, which wait on their own counter instead of
-// behind the previous item's vector stores.
-template <class T>
-__device__ __forceinline__ const __attribute__((address_space(4))) T* g_const(const T* p)
+// An item's description travels lane-packed in one VGPR: lanes 0..15 hold
+// the 16 words of its record, lanes 16..21 its A, B and C pointers (low word
+// first), lanes 32..47 the record of the item after it.  One vector load per
+// item fills it; v_readlane decodes it into scalars.
+__device__ __forceinline__ uint32_t g_fetch(const GroupedDevice& d, int it, int inst, int it_after, int lane)
 {
-    return (const __attribute__((address_space(4))) T*)(uintptr_t)p;
+    const int w = min(max(lane - 16, 0), 5);
+    const void* const* arr = w < 2 ? d.A : w < 4 ? d.B : const_cast<const void* const*>(d.C);
+    const uint32_t* ptr_word = reinterpret_cast<const uint32_t*>(arr + inst) + (w & 1);
+    const int r = (lane >= 32 && lane < 48) ? it_after : it;
+    const uint32_t* rec_word = reinterpret_cast<const uint32_t*>(d.recs + r) + (lane & (GREC_WORDS - 1));
+    return *gbl((lane >= 16 && lane < 22) ? ptr_word : rec_word);
 }
-// Element idx of a table, word by word through the constant address space.
-template <class T>
-__device__ __forceinline__ T g_load(const T* base, long idx)
+// A copy of v made once v's load has landed: a value carried to the next
+// item without a load in flight behind it, so using it there waits for
+// nothing (otherwise the wait, in order behind this item's stores, would
+// also wait for them).
+__device__ __forceinline__ uint32_t g_settle(uint32_t v)
 {
-    static_assert(sizeof(T) % 4 == 0, "whole words");
-    const __attribute__((address_space(4))) uint32_t* w = g_const(reinterpret_cast<const uint32_t*>(base + idx));
-    uint32_t tmp[sizeof(T) / 4];
-#pragma unroll
-    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) tmp[i] = w[i];
-    T out;
-    __builtin_memcpy(&out, tmp, sizeof(T));
-    return out;
+    uint32_t r;
+    __asm__ volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
 }
 
-__device__ __forceinline__ TileJob g_job(const GroupedDevice& d, const GItem& item, const GInst& in, const GGroup& g)
-{
+struct GWork {
     TileJob j;
-    j.A = static_cast<const char*>(g_const(d.A)[item.inst]);
-    j.B = static_cast<const char*>(g_const(d.B)[item.inst]);
-    j.C = g_const(d.C)[item.inst];
-    j.lda = in.lda;
-    j.ldb = in.ldb;
-    j.ldc = in.ldc;
-    j.m = in.m;
-    j.n = in.n;
-    j.k = in.k;
-    j.i0 = item.i0;
-    j.j0 = item.j0;
-    j.s = Scalars{g.s.af, g.s.bf, g.s.ad, g.s.bd, g.s.ai, g.s.bi};
-    j.beta0 = g.beta0;
-    j.akc = g.akc;
-    j.bkc = g.bkc;
-    return j;
+    int cfg, form;
+    uint32_t abytes, bbytes;
+};
+
+__device__ __forceinline__ GWork g_decode(uint32_t v)
+{
+    auto w = [&](int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); };
+    auto ptr = [&](int l) { return (uintptr_t)(((uint64_t)w(l + 1) << 32) | w(l)); };
+    GWork g;
+    g.j.A = reinterpret_cast<const char*>(ptr(16));
+    g.j.B = reinterpret_cast<const char*>(ptr(18));
+    g.j.C = reinterpret_cast<void*>(ptr(20));
+    g.j.i0 = (int)w(1);
+    g.j.j0 = (int)w(2);
+    const uint32_t bits = w(3);
+    g.j.m = (int)w(4);
+    g.j.n = (int)w(5);
+    g.j.k = (int)w(6);
+    g.j.lda = (int)w(7);
+    g.j.ldb = (int)w(8);
+    g.j.ldc = (int)w(9);
+    g.abytes = w(10);
+    g.bbytes = w(11);
+    const uint32_t a0 = w(12), a1 = w(13), b0 = w(14), b1 = w(15);
+    g.j.s.af = __builtin_bit_cast(float, a0);
+    g.j.s.bf = __builtin_bit_cast(float, b0);
+    g.j.s.ad = __builtin_bit_cast(double, ((uint64_t)a1 << 32) | a0);
+    g.j.s.bd = __builtin_bit_cast(double, ((uint64_t)b1 << 32) | b0);
+    g.j.s.ai = (int32_t)a0;
+    g.j.s.bi = (int32_t)b0;
+    g.cfg = (int)(bits & 3u);
+    g.form = (int)((bits >> 2) & 7u);
+    g.j.akc = (int)((bits >> 5) & 1u);
+    g.j.bkc = (int)((bits >> 6) & 1u);
+    g.j.beta0 = (int)((bits >> 7) & 1u);
+    return g;
+}
+
+// A workgroup's walk over the workgroup items: queue q = b % GQ, items
+// rank, rank + nq, rank + 2 nq, ... of it (nq workgroups take queue q, rank
+// = b / GQ); with fewer workgroups than queues, workgroup b takes queues b,
+// b + G, b + 2 G, ... (G = gridDim.x) on its own.  it < 0: no item left.
+struct GCursor {
+    int q, it, end, nq;
+};
+__device__ __forceinline__ void g_queue(const GroupedDevice& d, int q, GCursor& c)
+{
+    const int G = (int)gridDim.x;
+    c.q = q;
+    c.nq = G < GQ ? 1 : (G - q + GQ - 1) / GQ;
+    c.it = d.qoff[q] + (G < GQ ? 0 : (int)blockIdx.x / GQ);
+    c.end = d.qoff[q + 1];
+}
+__device__ __forceinline__ void g_skip(const GroupedDevice& d, GCursor& c)
+{
+    const int qstep = (int)gridDim.x < GQ ? (int)gridDim.x : GQ;
+    while (c.it >= 0 && c.it >= c.end) {
+        if (c.q + qstep >= GQ) c.it = -1;
+        else g_queue(d, c.q + qstep, c);
+    }
+}
+__device__ __forceinline__ void g_next(const GroupedDevice& d, GCursor& c)
+{
+    if (c.it < 0) return;
+    c.it += c.nq;
+    g_skip(d, c);
 }
 
 // FA: the family's first form; FB: its second (== FA for one-form families).
 // Workgroup items first (the longer ones): workgroup b, on XCD b % 8, takes
-// the items of queue q = b % GQ in turn with the other workgroups of that
-// queue (rank r = b / GQ of nq: items r, r + nq, r + 2 nq, ... -- the queue
-// is sorted longest first, so every workgroup gets one item of each stretch
-// of similar cost).  Then wave items, the same way over all the grid's
-// waves.  The assignment is fixed: no claims, so nothing waits on an atomic
-// (whose return, on gfx9, retires in order behind the previous tile's
-// stores) or on the barriers that shared a claimed number.
+// its fixed share of queue b % GQ (the queue is sorted longest first, so
+// every workgroup gets one item of each stretch of similar cost).  Then
+// wave items, the same way over all the grid's waves.  The assignment is
+// fixed: no claims, so nothing waits on an atomic (whose return, on gfx9,
+// retires in order behind the previous tile's stores) or on the barriers
+// that shared a claimed number.
+// While an item runs, the record of the item after next and the pointers of
+// the next item are read; they land with the item's first k-block, so an
+// item starts with its description at hand instead of a chain of dependent
+// loads, each in order behind the previous item's stores.
 template <class FA, class FB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) grouped_gemm(const GroupedDevice d)
 {
     constexpr int LA = g_lds_max<FA>(), LB = g_lds_max<FB>();
     __shared__ __attribute__((aligned(16))) char smem[LA > LB ? LA : LB];
-    const int tid = threadIdx.x;
-    // with fewer workgroups than queues, workgroup b also takes queues
-    // b + G, b + 2 G, ... (G = gridDim.x) on its own
-    const int G = (int)gridDim.x;
-    const int qstep = G < GQ ? G : GQ;
-    for (int q = (int)blockIdx.x % GQ; q < GQ; q += qstep) {
-        const int nq = G < GQ ? 1 : (G - q + GQ - 1) / GQ;  // workgroups taking queue q
-        const int rank = G < GQ ? 0 : (int)blockIdx.x / GQ;
-        for (int it = d.qoff[q] + rank; it < d.qoff[q + 1]; it += nq) {
-            const GItem item = g_load(d.items, it);
-            const GInst in = g_load(d.insts, item.inst);
-            const GGroup g = g_load(d.groups, in.group);
-            const TileJob j = g_job(d, item, in, g);
-            if (g.form == FA::ID) g_run<FA>(item.cfg, j, in.abytes, in.bbytes, smem);
-            else g_run<FB>(item.cfg, j, in.abytes, in.bbytes, smem);
+    const int lane = threadIdx.x & 63;
+    if (d.nitems > 0) {
+        GCursor c0;
+        g_queue(d, (int)blockIdx.x % GQ, c0);
+        g_skip(d, c0);
+        if (c0.it >= 0) {
+            GCursor c1 = c0;
+            g_next(d, c1);
+            GCursor c2 = c1;
+            g_next(d, c2);
+            auto valid = [&](const GCursor& c) { return c.it >= 0 ? c.it : c0.it; };
+            // the first item's record, then its pointers with the next record
+            const uint32_t r0 = g_fetch(d, c0.it, 0, c0.it, lane);
+            uint32_t v = g_settle(g_fetch(d, c0.it, __builtin_amdgcn_readlane((int)r0, 0), valid(c1), lane));
+            while (c0.it >= 0) {
+                const GWork w = g_decode(v);
+                const uint32_t t = g_fetch(d, valid(c1), __builtin_amdgcn_readlane((int)v, 32), valid(c2), lane);
+                uint32_t v_next = 0u;
+                auto prefetched = [&]() { v_next = g_settle(t); };
+                if (w.form == FA::ID) g_run<FA>(w.cfg, w.j, w.abytes, w.bbytes, smem, prefetched);
+                else g_run<FB>(w.cfg, w.j, w.abytes, w.bbytes, smem, prefetched);
+                v = v_next;
+                c0 = c1;
+                c1 = c2;
+                g_next(d, c2);
+            }
         }
     }
     if (d.nwitems > 0) {
-        const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
         const int nw = 4 * (int)gridDim.x;
-        for (int w = 4 * (int)blockIdx.x + wave; w < d.nwitems; w += nw) {
-            const GItem item = g_load(d.items, d.nitems + w);
-            const GInst in = g_load(d.insts, item.inst);
-            const GGroup g = g_load(d.groups, in.group);
-            const TileJob j = g_job(d, item, in, g);
-            if (g.form == FA::ID) g_run_wave<FA>(j);
-            else g_run_wave<FB>(j);
+        const int w0 = 4 * (int)blockIdx.x + wave;
+        if (w0 < d.nwitems) {
+            auto idx = [&](long w) { return d.nitems + (w < d.nwitems ? (int)w : w0); };
+            const uint32_t r0 = g_fetch(d, idx(w0), 0, idx(w0), lane);
+            uint32_t v = g_settle(g_fetch(d, idx(w0), __builtin_amdgcn_readlane((int)r0, 0), idx((long)w0 + nw), lane));
+            for (int w = w0; w < d.nwitems; w += nw) {
+                const GWork g = g_decode(v);
+                const uint32_t t = g_fetch(d, idx((long)w + nw), __builtin_amdgcn_readlane((int)v, 32),
+                                           idx((long)w + 2 * (long)nw), lane);
+                if (g.form == FA::ID) g_run_wave<FA>(g.j);
+                else g_run_wave<FB>(g.j);
+                v = g_settle(t);
+            }
         }
     }
 }

@@ -109,6 +109,25 @@ def test_grouped_edges():
     check(G, got)
 
 
+@pytest.mark.parametrize("cfg", [None, "g32", "g64", "g128"])
+def test_grouped_matrix_ends(cfg, monkeypatch):
+    """Leading dimensions equal to the stored rows (every matrix ends exactly
+    at its last element, no padding after it) with tiny k, one-row and
+    one-column operands: the word that holds an int8 matrix's last bytes
+    falls in an earlier row tile or an earlier k-block than the last one,
+    and k tails of K-contiguous operands read the next row's data."""
+    if cfg is not None:
+        monkeypatch.setenv("SSBLAS_GROUPED_CFG", cfg)
+    dims = [(33, 5, 1), (33, 33, 2), (1, 3, 65), (1, 1, 129), (2, 70, 65), (3, 3, 3), (31, 2, 17), (65, 1, 1),
+            (129, 130, 1), (2, 2, 64), (3, 1, 128), (70, 3, 2), (1, 65, 63), (5, 5, 129)]
+    dims = dims * 4
+    G = make_grouped(len(dims), seed=0x55B1A5 + 57, group=1, dims=dims, pad=0, types=("i8", "f32"))
+    dg = DeviceGrouped(G)
+    with plan_for(dg) as p:
+        assert p.execute() == 0
+        check(G, dg.results())
+
+
 @pytest.mark.parametrize("cfg", [None, "w32", "w32:g64"])
 def test_grouped_fp64_and_bf16x3_families(cfg, monkeypatch):
     if cfg is not None:
