@@ -1053,12 +1053,117 @@ __device__ __forceinline__ void oz_small_load(const void* X, const OzLane& L, do
     }
 }
 
+// One instance of the fused form once its operands are in registers (lane l:
+// row l % 32 of op(A), column l % 32 of op(B), k half l / 32): exponents,
+// slices, the triangle of slice products on v_mfma_i32_32x32x32_i8, the fold
+// (smallest order first), the epilogue into C and the error report.
+// before_epilogue() runs between the products and the first access to C.
+template <int S, class Hook>
+__device__ __forceinline__ void oz_small_instance(const OzParams& p, long inst, const double (&xa)[16],
+                                                  const double (&xb)[16], double* C, int lane,
+                                                  Hook&& before_epilogue)
+{
+    const int own = lane & 31, kh = lane >> 5;
+    const bool beta0 = p.beta0 != 0;
+    u32x4 sa[S], sb[S];
+    OzStats sta, stb;
+    uint32_t dsum[S];
+#pragma unroll
+    for (int q = 0; q < S; ++q) dsum[q] = 0u;
+    const int ea = oz_exp16(xa);
+    oz_slice_regs<S, true, true>(xa, ea, sa, &sta, dsum);
+    const int eb = oz_exp16(xb);
+    oz_slice_regs<S, true, false>(xb, eb, sb, &stb);
+
+    double D[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) D[v] = 0.0;
+#pragma unroll
+    for (int t = S + 1; t >= 2; --t) {
+        const int plo = t - S > 1 ? t - S : 1, phi = S < t - 1 ? S : t - 1;
+        i32x16 acc = {};
+#pragma unroll
+        for (int pa = plo; pa <= phi; ++pa)
+            acc = __builtin_amdgcn_mfma_i32_32x32x32_i8((i32x4)sa[pa - 1], (i32x4)sb[t - pa - 1], acc, 0, 0, 0);
+        const double scale = ldexp(1.0, -7 * t);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) D[v] = __builtin_fma((double)acc[v], scale, D[v]);
+        // fold each order here: left alone, the compiler sinks the folds
+        // into the epilogue and keeps S accumulators of 16 registers live
+#pragma unroll
+        for (int v = 0; v < 16; ++v) __asm__ volatile("" : "+v"(D[v]));
+    }
+    before_epilogue();
+
+    // epilogue: C = fma(D * 2^(e_i + f_j), alpha, fl(beta * C)).  Lane
+    // (own, kh) holds rows 8 q + 4 kh .. + 3 of column own, q = 0..3.  With
+    // all 32 rows inside the instance and C 16-byte aligned, they go out as
+    // 16-byte pairs, beta == 0 and beta != 0 decided once for the wave and
+    // the old values read before the first store: a load issued
+    // conditionally between the stores puts a vmcnt wait at its join, and
+    // on gfx9 that wait also waits for every store issued before it.
+    const int c = own;
+    // row exponents, shuffled while every lane is active (lane `row`
+    // holds that row's)
+    int eir[16];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) eir[v] = __shfl(ea, 8 * (v >> 2) + 4 * kh + (v & 3));
+    if (c < p.n) {
+        double val[16];
+#pragma unroll
+        for (int v = 0; v < 16; ++v)
+            val[v] = (eir[v] == OZ_NONFINITE || eb == OZ_NONFINITE) ? __builtin_nan("") : ldexp(D[v], eir[v] + eb);
+        double* col = C + (long)c * p.ldc + 4 * kh;
+        if ((p.ldc % 2) == 0 && aligned16(C) && p.m >= 32) {
+            if (beta0) {
+#pragma unroll
+                for (int h = 0; h < 8; ++h) {
+                    f64x2 res;
+                    res[0] = __builtin_fma(val[2 * h], p.sc.ad, 0.0);
+                    res[1] = __builtin_fma(val[2 * h + 1], p.sc.ad, 0.0);
+                    *gbl(reinterpret_cast<f64x2*>(col + 8 * (h >> 1) + 2 * (h & 1))) = res;
+                }
+            } else {
+                f64x2 old[8];
+#pragma unroll
+                for (int h = 0; h < 8; ++h) old[h] = *gbl(reinterpret_cast<const f64x2*>(col + 8 * (h >> 1) + 2 * (h & 1)));
+#pragma unroll
+                for (int h = 0; h < 8; ++h) {
+                    f64x2 res;
+                    res[0] = __builtin_fma(val[2 * h], p.sc.ad, old[h][0] * p.sc.bd);
+                    res[1] = __builtin_fma(val[2 * h + 1], p.sc.ad, old[h][1] * p.sc.bd);
+                    *gbl(reinterpret_cast<f64x2*>(col + 8 * (h >> 1) + 2 * (h & 1))) = res;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = 8 * (v >> 2) + 4 * kh + (v & 3);
+                if (row < p.m) {
+                    double* dst = col + 8 * (v >> 2) + (v & 3);
+                    const double t0 = beta0 ? 0.0 : (*gbl(dst)) * p.sc.bd;
+                    *gbl(dst) = __builtin_fma(val[v], p.sc.ad, t0);
+                }
+            }
+        }
+    }
+    // the instance's error bound, from the statistics gathered while
+    // slicing (lanes beyond the instance hold zeros); computed after the
+    // stores so that its reductions do not add to the product loop's
+    // registers
+    if (p.bound != nullptr) {
+        const float bound = oz_bound<S>(ea, eb, sta, stb, dsum);
+        if (lane == 0) {
+            p.bound[inst] = bound;
+            p.sarr[inst] = (int8_t)S;
+        }
+    }
+}
+
 // Persistent: wave w of the grid computes instances w, w + W, w + 2W, ...
 // (W = waves in the grid), loading each instance's operands straight into
-// registers.  (Copying the next instance global -> LDS while computing the
-// current one was measured slower: the extra LDS reads and waits cost more
-// than the load latency they hide at two waves per SIMD.)
-template <int S, bool FULL>
+// registers (instances smaller than 32 x 32 x 32: guarded loads).
+template <int S>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) ozaki_small(const OzParams p)
 {
     const int lane = threadIdx.x & 63;
@@ -1069,107 +1174,161 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
     long inst = (long)blockIdx.x * 4 + wave;
     if (inst >= p.batch) return;
     const int own = lane & 31, kh = lane >> 5;
-    const bool beta0 = p.beta0 != 0;
     // op(A) row own is K-contiguous when A is transposed; op(B) column own
     // when B is not
     const OzLane LA = oz_lane(p.lda, p.ta != 0, p.m, p.k, own, kh);
     const OzLane LB = oz_lane(p.ldb, p.tb == 0, p.n, p.k, own, kh);
-
     for (; inst < p.batch; inst += W) {
         double xa[16], xb[16];
-        oz_small_load<FULL>(p.A[inst], LA, xa);
-        oz_small_load<FULL>(p.B[inst], LB, xb);
-        u32x4 sa[S], sb[S];
-        OzStats sta, stb;
-        uint32_t dsum[S];
-#pragma unroll
-        for (int q = 0; q < S; ++q) dsum[q] = 0u;
-        const int ea = oz_exp16(xa);
-        oz_slice_regs<S, true, true>(xa, ea, sa, &sta, dsum);
-        const int eb = oz_exp16(xb);
-        oz_slice_regs<S, true, false>(xb, eb, sb, &stb);
+        oz_small_load<false>(p.A[inst], LA, xa);
+        oz_small_load<false>(p.B[inst], LB, xb);
+        oz_small_instance<S>(p, inst, xa, xb, wave_uniform(static_cast<double*>(p.C[inst])), lane, [] {});
+    }
+}
 
-        double D[16];
-#pragma unroll
-        for (int v = 0; v < 16; ++v) D[v] = 0.0;
-#pragma unroll
-        for (int t = S + 1; t >= 2; --t) {
-            const int plo = t - S > 1 ? t - S : 1, phi = S < t - 1 ? S : t - 1;
-            i32x16 acc = {};
-#pragma unroll
-            for (int pa = plo; pa <= phi; ++pa)
-                acc = __builtin_amdgcn_mfma_i32_32x32x32_i8((i32x4)sa[pa - 1], (i32x4)sb[t - pa - 1], acc, 0, 0, 0);
-            const double scale = ldexp(1.0, -7 * t);
-#pragma unroll
-            for (int v = 0; v < 16; ++v) D[v] = __builtin_fma((double)acc[v], scale, D[v]);
-            // fold each order here: left alone, the compiler sinks the folds
-            // into the epilogue and keeps S accumulators of 16 registers live
-#pragma unroll
-            for (int v = 0; v < 16; ++v) __asm__ volatile("" : "+v"(D[v]));
-        }
+// ---------------------------------------------------------------------------
+// Whole 32 x 32 x 32 instances: each wave owns a 16 KiB LDS image of one
+// instance (op(A), then op(B), 8 KiB each) and fills it with
+// global_load_lds_dwordx4 (1 KiB per wave instruction, no registers): as soon
+// as it has read the current instance out of the image into registers, it
+// requests the next instance's operands into it, so their loads are in
+// flight while the current instance is sliced and multiplied (the register
+// form above waits for every instance's loads in turn, at the two waves per
+// SIMD its registers allow).
+//
+// Image of one operand, 16-byte chunks (two doubles):
+//   M/N-contiguous (element (r, k) at r + k ld): [k][row], chunk c of k-row
+//     kk at slot 16 kk + c; the lane reads its 16 values with ds_read_b64;
+//   K-contiguous (element (r, k) at r ld + k): [row][k], chunk c of row r at
+//     slot 16 r + (c ^ (r & 15)), so the 32 rows a read instruction covers
+//     fall on 16 different 16-byte bank groups; ds_read_b128.
+// 16-byte loads need 16-byte aligned chunks (even leading dimension, A or B
+// 16-byte aligned); otherwise the operand's image is filled with 4-byte
+// loads (global_load_lds_dword, 256 bytes per wave instruction) into the
+// same layout.
+//
+// Pointer words are read two instances ahead (lane-packed in one VGPR) and
+// taken into use once they have landed, before the epilogue.
+// Waits: the image's loads are not tracked by the compiler.  Behind the
+// loads of an instance's image the wave issues at least nine more vector
+// memory instructions before it reads the image (the pointer words of the
+// instance after, and at least eight C stores of a 32 x 32 fp64 block), so
+// vmcnt(9) there means the image is complete; the first image is waited
+// for in full.
+// ---------------------------------------------------------------------------
+constexpr int OZ_IMG = 32 * 32 * 8;  // one operand
 
-        // epilogue: C = fma(D * 2^(e_i + f_j), alpha, fl(beta * C)); the
-        // lane's 4 consecutive rows of its column go out as two 16-byte
-        // stores when they are whole and aligned
-        double* C = wave_uniform(static_cast<double*>(p.C[inst]));
-        const bool vec = (p.ldc % 2) == 0 && aligned16(C);
-        const int c = own;
-        // row exponents, shuffled while every lane is active (lane `row`
-        // holds that row's)
-        int eir[16];
+// The lane index goes through an empty asm in these helpers: left visible,
+// the compiler hoists every lane-dependent address out of the instance loop
+// and keeps them all live in registers across it.
+__device__ __forceinline__ void oz_dma_operand(const double* X, long ld, bool kc, char* img, int lane_in)
+{
+    int lane = lane_in;
+    __asm__ volatile("" : "+v"(lane));
+    const char* base = reinterpret_cast<const char*>(X);
+    if (((ld % 2) == 0) && aligned16(X)) {
 #pragma unroll
-        for (int v = 0; v < 16; ++v) eir[v] = __shfl(ea, 8 * (v >> 2) + 4 * kh + (v & 3));
-        if (c < p.n) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int row0 = 8 * q + 4 * kh;
-                double val[4];
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int ei = eir[4 * q + v];
-                    val[v] = (ei == OZ_NONFINITE || eb == OZ_NONFINITE) ? __builtin_nan("")
-                                                                        : ldexp(D[4 * q + v], ei + eb);
-                }
-                double* dst = C + (long)c * p.ldc + row0;
-                if (vec && row0 + 4 <= p.m) {
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        f64x2 old = {0.0, 0.0};
-                        if (!beta0) old = *gbl(reinterpret_cast<const f64x2*>(dst + 2 * h));
-                        f64x2 res;
-                        res[0] = __builtin_fma(val[2 * h], p.sc.ad, beta0 ? 0.0 : old[0] * p.sc.bd);
-                        res[1] = __builtin_fma(val[2 * h + 1], p.sc.ad, beta0 ? 0.0 : old[1] * p.sc.bd);
-                        *gbl(reinterpret_cast<f64x2*>(dst + 2 * h)) = res;
-                    }
-                } else {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        if (row0 + v < p.m) {
-                            const double t0 = beta0 ? 0.0 : (*gbl(dst + v)) * p.sc.bd;
-                            *gbl(dst + v) = __builtin_fma(val[v], p.sc.ad, t0);
-                        }
-                    }
-                }
-            }
+        for (int q = 0; q < 8; ++q) {
+            const int slot = 64 * q + lane;
+            const int hi = slot >> 4, c = slot & 15;  // row (KC) or k (M/N) of the slot, and its chunk
+            const long off = (long)hi * ld + 2 * (kc ? (c ^ (hi & 15)) : c);
+            __builtin_amdgcn_global_load_lds(base + 8 * off, (lds_void_t*)(img + 1024 * q), 16, 0, 0);
         }
-        // the instance's error bound, from the statistics gathered while
-        // slicing (lanes beyond the instance hold zeros); computed after the
-        // stores so that its reductions do not add to the product loop's
-        // registers
-        if (p.bound != nullptr) {
-            const float bound = oz_bound<S>(ea, eb, sta, stb, dsum);
-            if (lane == 0) {
-                p.bound[inst] = bound;
-                p.sarr[inst] = (int8_t)S;
-            }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 32; ++q) {
+            const int word = 64 * q + lane;  // 4-byte word of the image
+            const int slot = word >> 2, hi = slot >> 4, c = slot & 15;
+            const long off = (long)hi * ld + 2 * (kc ? (c ^ (hi & 15)) : c);
+            __builtin_amdgcn_global_load_lds(base + 8 * off + 4 * (word & 3), (lds_void_t*)(img + 256 * q), 4, 0, 0);
         }
+    }
+}
+
+__device__ __forceinline__ void oz_img_read(const char* img, bool kc, int lane_in, double (&x)[16])
+{
+    int lane = lane_in;
+    __asm__ volatile("" : "+v"(lane));
+    const int own = lane & 31, kh = lane >> 5;
+    if (kc) {
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const int c = 8 * kh + h;
+            const f64x2 v = *reinterpret_cast<const f64x2*>(img + 256 * own + 16 * (c ^ (own & 15)));
+            x[2 * h] = v[0];
+            x[2 * h + 1] = v[1];
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) x[q] = *reinterpret_cast<const double*>(img + 256 * (16 * kh + q) + 8 * own);
+    }
+}
+
+// Pointer words of instance `inst`, lane-packed: lanes 0-1 A, 2-3 B, 4-5 C
+// (low word first); the other lanes read C's high word again.
+__device__ __forceinline__ uint32_t oz_ptr_word(const OzParams& p, long inst, int lane)
+{
+    const int w = lane < 5 ? lane : 5;
+    const void* const* arr = w < 2 ? p.A : w < 4 ? p.B : const_cast<const void* const*>(p.C);
+    return *gbl(reinterpret_cast<const uint32_t*>(arr + inst) + (w & 1));
+}
+__device__ __forceinline__ const void* oz_ptr_of(uint32_t v, int which)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)v, 2 * which);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)v, 2 * which + 1);
+    return reinterpret_cast<const void*>(((uint64_t)hi << 32) | lo);
+}
+// A copy of v once its load has landed (carried across iterations with no
+// load in flight behind it)
+__device__ __forceinline__ uint32_t oz_settle(uint32_t v)
+{
+    uint32_t r;
+    __asm__ volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) ozaki_small_full(const OzParams p)
+{
+    __shared__ __attribute__((aligned(1024))) char smem[4 * 2 * OZ_IMG];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long W = (long)gridDim.x * 4;
+    long inst = (long)blockIdx.x * 4 + wave;
+    if (inst >= p.batch) return;
+    const bool akc = p.ta != 0, bkc = p.tb == 0;
+    char* img = smem + wave * 2 * OZ_IMG;
+    auto dma = [&](uint32_t pw) {
+        oz_dma_operand(static_cast<const double*>(oz_ptr_of(pw, 0)), p.lda, akc, img, lane);
+        oz_dma_operand(static_cast<const double*>(oz_ptr_of(pw, 1)), p.ldb, bkc, img + OZ_IMG, lane);
+    };
+    uint32_t pw = oz_settle(oz_ptr_word(p, inst, lane));
+    uint32_t pwn = oz_settle(oz_ptr_word(p, inst + W < p.batch ? inst + W : inst, lane));
+    dma(pw);
+    __asm__ __volatile__("s_waitcnt vmcnt(0)" ::: "memory");
+    for (; inst < p.batch; inst += W) {
+        const long nxt = inst + W, after = inst + 2 * W;
+        double xa[16], xb[16];
+        __asm__ __volatile__("s_waitcnt vmcnt(9)" ::: "memory");
+        oz_img_read(img, akc, lane, xa);
+        oz_img_read(img + OZ_IMG, bkc, lane, xb);
+        // read out before the next instance's loads overwrite the image
+        __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
+        double* C = static_cast<double*>(const_cast<void*>(oz_ptr_of(pw, 2)));
+        if (nxt < p.batch) dma(pwn);
+        const uint32_t t = oz_ptr_word(p, after < p.batch ? after : inst, lane);
+        uint32_t pwa = 0u;
+        oz_small_instance<S>(p, inst, xa, xb, C, lane, [&] { pwa = oz_settle(t); });
+        pw = pwn;
+        pwn = pwa;
     }
 }
 
 template <int S, bool FULL>
 static hipError_t run_ozaki_small_k(const OzParams& p, hipStream_t st)
 {
-    const int occ = kernel_occupancy(ozaki_small<S, FULL>, 256);
+    auto kern = FULL ? ozaki_small_full<S> : ozaki_small<S>;
+    const int occ = kernel_occupancy(kern, 256);
     const long blocks = ((long)p.batch + 3) / 4;
     long grid = (long)occ * device_cu_count();
     if (const char* e = getenv("SSBLAS_MAX_GRID")) {
@@ -1177,7 +1336,7 @@ static hipError_t run_ozaki_small_k(const OzParams& p, hipStream_t st)
         if (cap >= 1 && cap < grid) grid = cap;
     }
     if (blocks < grid) grid = blocks;
-    hipLaunchKernelGGL((ozaki_small<S, FULL>), dim3((unsigned)grid), dim3(256), 0, st, p);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, p);
     return hipGetLastError();
 }
 
