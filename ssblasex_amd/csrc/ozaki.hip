@@ -851,14 +851,16 @@ __device__ __forceinline__ void oz_slice_regs(const double* x, int e, u32x4 (&sl
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int i = 4 * j + q;
-            const double a = (e == OZ_NONFINITE) ? 0.0 : ldexp(fabs(x[i]), -e);
+            // |x| 2^-e 2^28 in one scaling (where |x| 2^-e would be
+            // subnormal every digit is zero either way); the remainders by
+            // v_fract_f64, exact for these non-negative values below 2^28
+            const double y1 = (e == OZ_NONFINITE) ? 0.0 : ldexp(fabs(x[i]), 28 - e);
             neg |= (x[i] < 0.0 ? 0xffu : 0u) << (8 * q);
-            const double y1 = ldexp(a, 28);
             H[q] = (uint32_t)y1;
             if constexpr (S > 4) {
-                const double y2 = ldexp(y1 - (double)H[q], 28);
+                const double y2 = ldexp(__builtin_amdgcn_fract(y1), 28);
                 L[q] = (uint32_t)y2;
-                if constexpr (S > 8) N[q] = (uint32_t)ldexp(y2 - (double)L[q], 7);
+                if constexpr (S > 8) N[q] = (uint32_t)ldexp(__builtin_amdgcn_fract(y2), 7);
             }
         }
         uint32_t w[9];
@@ -989,20 +991,25 @@ __device__ __forceinline__ float oz_bound(int ea, int eb, OzStats sa, OzStats sb
 }
 
 
-// Exponent of the max |.| over both k halves (lanes l and l ^ 32).
+// Exponent of the max |.| over both k halves (lanes l and l ^ 32), as
+// exp_of_max gives it: from the largest biased exponent field E of the
+// values (the high words of |x| compare as the values do), e = E - 1022 for
+// a normal maximum, OZ_NONFINITE for E = 0x7ff (inf / nan); a maximum that
+// is zero or subnormal (E = 0) goes through frexp.
 __device__ __forceinline__ int oz_exp16(const double* x)
 {
-    double mx = 0.0;
-    bool bad = false;
+    uint32_t hmax = 0u;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const double a = fabs(x[i]);
-        bad |= !isfinite(a);
-        mx = fmax(mx, a);
-    }
+    for (int i = 0; i < 16; ++i) hmax = max(hmax, (uint32_t)(__builtin_bit_cast(uint64_t, x[i]) >> 32) & 0x7fffffffu);
+    hmax = max(hmax, (uint32_t)__shfl_xor((int)hmax, 32));
+    const int E = (int)(hmax >> 20);
+    if (E == 0x7ff) return OZ_NONFINITE;
+    if (E > 0) return E - 1022;
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) mx = fmax(mx, fabs(x[i]));
     mx = fmax(mx, __shfl_xor(mx, 32));
-    bad |= __shfl_xor((int)bad, 32) != 0;
-    return exp_of_max(mx, bad);
+    return exp_of_max(mx, false);
 }
 
 // Where a lane's 16 doubles sit in one operand: row `own` of op(A) (column

@@ -74,6 +74,37 @@ def test_ozaki_nonfinite_rows_become_nan():
     assert np.isfinite(np.delete(out, 3, axis=0)).all()
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 1), (1, 0)])
+def test_ozaki_fused_subnormal_zero_and_nonfinite_rows(ta, tb, monkeypatch):
+    """Whole 32^3 instances whose rows of op(A) / columns of op(B) have a
+    subnormal maximum (the row exponent then comes from frexp, not from the
+    biased exponent field), are all zero, mix subnormal and normal values,
+    or hold an infinity: the fused kernel gives the general path's result
+    bit for bit (NaN where an infinity reaches)."""
+    b = make_batch("f64", ta, tb, 32, 32, 32, 6, alpha=1.0, beta=0.0, seed=61, dist="wide")
+    for i in range(6):
+        A = oracle.colmajor(b.A[i], *oracle.op_dims(ta, 32, 32), b.lda)
+        B = oracle.colmajor(b.B[i], *oracle.op_dims(tb, 32, 32), b.ldb)
+        opA = A.T if ta else A      # views: writes land in the stored matrix
+        opB = B.T if tb else B
+        opA[2 * i] *= 2.0 ** -1060          # a row of subnormals
+        opA[2 * i + 1] = 0.0                # an all-zero row
+        opA[2 * i + 3, :5] *= 2.0 ** -1065  # a row mixing subnormal and normal values
+        opB[:, 3 * i] *= 2.0 ** -1040       # a column of subnormals
+        opB[:, 3 * i + 1] = 0.0
+        if i == 4:
+            opA[7, 9] = np.inf
+    st, fused = run_gpu(b, compute_type=0x109)
+    assert st == 0
+    monkeypatch.setenv("SSBLAS_OZAKI_NO_FUSED", "1")
+    st, general = run_gpu(b, compute_type=0x109)
+    assert st == 0
+    for i, (x, y) in enumerate(zip(fused, general)):
+        assert np.array_equal(x, y, equal_nan=True), f"instance {i}"
+    out = oracle.colmajor(fused[4], 32, 32, b.ldc)
+    assert np.isnan(out[7]).all()
+
+
 def test_ozaki_long_k_chunked():
     """K beyond one int32-exact chunk (8192) is folded chunk by chunk."""
     b = make_batch("f64", 1, 0, 8, 8, 9000, 1, alpha=1.0, beta=0.0, seed=5)
