@@ -940,18 +940,40 @@ __device__ __forceinline__ void oz_slice_regs(const double* x, int e, u32x4 (&sl
 // ---------------------------------------------------------------------------
 constexpr float OZ_TARGET = 2.7755575615628914e-17f;  // 2^-55, a quarter unit roundoff of fp64
 
-// Sum over the 32 row (column) owners; lanes l and l ^ 32 hold the same values.
+// Reductions over the 32 row (column) owners (lanes l and l ^ 32 hold the
+// same values) without LDS-routed shuffles: within each 16-lane DPP row by
+// quad swaps (xor 1, xor 2) and row rotations by 4 and 8 (every lane then
+// holds its row's total), across the two rows of a half by ds_swizzle
+// (xor 16).  The joins of the two k halves (xor 32) use v_permlane32_swap.
+template <class Op>
+__device__ __forceinline__ uint32_t oz_row16(uint32_t v, Op op)
+{
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false));  // row_ror:4
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false));  // row_ror:8
+    return op(v, (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F));                    // xor 16
+}
 __device__ __forceinline__ float oz_sum32(float v)
 {
-#pragma unroll
-    for (int off = 16; off > 0; off >>= 1) v += __shfl_xor(v, off);
-    return v;
+    return __builtin_bit_cast(float, oz_row16(__builtin_bit_cast(uint32_t, v), [](uint32_t a, uint32_t b) {
+        return __builtin_bit_cast(uint32_t, __builtin_bit_cast(float, a) + __builtin_bit_cast(float, b));
+    }));
 }
 __device__ __forceinline__ int oz_max32(int v)
 {
-#pragma unroll
-    for (int off = 16; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
-    return v;
+    return (int)oz_row16((uint32_t)v, [](uint32_t a, uint32_t b) { return (uint32_t)max((int)a, (int)b); });
+}
+// The value of lane l ^ 32.
+__device__ __forceinline__ uint32_t oz_other_half(uint32_t v)
+{
+    const auto pr = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    // pr[0]: the lower half's values in both halves, pr[1]: the upper half's
+    return (threadIdx.x & 32) ? pr[0] : pr[1];
+}
+__device__ __forceinline__ float oz_other_half(float v)
+{
+    return __builtin_bit_cast(float, oz_other_half(__builtin_bit_cast(uint32_t, v)));
 }
 
 // Bound at S slices for the fused one-wave form: lane l owns row l % 32 of
@@ -966,14 +988,14 @@ __device__ __forceinline__ float oz_bound(int ea, int eb, OzStats sa, OzStats sb
     if (__any(ea == OZ_NONFINITE || eb == OZ_NONFINITE)) return __builtin_huge_valf();
     // join the two k halves
     float a1 = sa.l1(), a2 = sa.l2(), b1 = sb.l1(), b2 = sb.l2();
-    a1 += __shfl_xor(a1, 32);
-    a2 += __shfl_xor(a2, 32);
-    b1 += __shfl_xor(b1, 32);
-    b2 += __shfl_xor(b2, 32);
+    a1 += oz_other_half(a1);
+    a2 += oz_other_half(a2);
+    b1 += oz_other_half(b1);
+    b2 += oz_other_half(b2);
     uint32_t Sa = 0;  // digit sum S_i(S)
 #pragma unroll
     for (int p = 0; p < S; ++p) Sa += dsum[p];
-    Sa += (uint32_t)__shfl_xor((int)Sa, 32);
+    Sa += oz_other_half(Sa);
     const bool rowz = a1 == 0.f, colz = b1 == 0.f;
     const int emax = oz_max32(rowz ? INT32_MIN : ea), fmax = oz_max32(colz ? INT32_MIN : eb);
     if (emax == INT32_MIN || fmax == INT32_MIN) return 0.f;  // op(A) or op(B) all zero: exact zero product
@@ -1001,7 +1023,7 @@ __device__ __forceinline__ int oz_exp16(const double* x)
     uint32_t hmax = 0u;
 #pragma unroll
     for (int i = 0; i < 16; ++i) hmax = max(hmax, (uint32_t)(__builtin_bit_cast(uint64_t, x[i]) >> 32) & 0x7fffffffu);
-    hmax = max(hmax, (uint32_t)__shfl_xor((int)hmax, 32));
+    hmax = max(hmax, oz_other_half(hmax));
     const int E = (int)(hmax >> 20);
     if (E == 0x7ff) return OZ_NONFINITE;
     if (E > 0) return E - 1022;
@@ -1065,9 +1087,10 @@ __device__ __forceinline__ void oz_small_load(const void* X, const OzLane& L, do
 // slices, the triangle of slice products on v_mfma_i32_32x32x32_i8, the fold
 // (smallest order first), the epilogue into C and the error report.
 // before_epilogue() runs between the products and the first access to C.
+// xch: the wave's 32 ints of LDS for the row exponents.
 template <int S, class Hook>
 __device__ __forceinline__ void oz_small_instance(const OzParams& p, long inst, const double (&xa)[16],
-                                                  const double (&xb)[16], double* C, int lane,
+                                                  const double (&xb)[16], double* C, int lane, int* xch,
                                                   Hook&& before_epilogue)
 {
     const int own = lane & 31, kh = lane >> 5;
@@ -1110,11 +1133,16 @@ __device__ __forceinline__ void oz_small_instance(const OzParams& p, long inst, 
     // conditionally between the stores puts a vmcnt wait at its join, and
     // on gfx9 that wait also waits for every store issued before it.
     const int c = own;
-    // row exponents, shuffled while every lane is active (lane `row`
-    // holds that row's)
+    // row exponents through LDS (lane `row` holds that row's; lanes l and
+    // l + 32 write the same value): four consecutive rows per 16-byte read
+    xch[own] = ea;
     int eir[16];
 #pragma unroll
-    for (int v = 0; v < 16; ++v) eir[v] = __shfl(ea, 8 * (v >> 2) + 4 * kh + (v & 3));
+    for (int q = 0; q < 4; ++q) {
+        const i32x4 r = *reinterpret_cast<const i32x4*>(xch + 8 * q + 4 * kh);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) eir[4 * q + j] = r[j];
+    }
     if (c < p.n) {
         double val[16];
 #pragma unroll
@@ -1185,11 +1213,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
     // when B is not
     const OzLane LA = oz_lane(p.lda, p.ta != 0, p.m, p.k, own, kh);
     const OzLane LB = oz_lane(p.ldb, p.tb == 0, p.n, p.k, own, kh);
+    __shared__ __attribute__((aligned(16))) int xch[4 * 32];
     for (; inst < p.batch; inst += W) {
         double xa[16], xb[16];
         oz_small_load<false>(p.A[inst], LA, xa);
         oz_small_load<false>(p.B[inst], LB, xb);
-        oz_small_instance<S>(p, inst, xa, xb, wave_uniform(static_cast<double*>(p.C[inst])), lane, [] {});
+        oz_small_instance<S>(p, inst, xa, xb, wave_uniform(static_cast<double*>(p.C[inst])), lane, xch + 32 * wave,
+                             [] {});
     }
 }
 
@@ -1234,12 +1264,17 @@ __device__ __forceinline__ void oz_dma_operand(const double* X, long ld, bool kc
     __asm__ volatile("" : "+v"(lane));
     const char* base = reinterpret_cast<const char*>(X);
     if (((ld % 2) == 0) && aligned16(X)) {
+        // slot 64 q + lane: row (KC) or k (M/N) 4 q + (lane >> 4), chunk
+        // lane & 15 (KC: stored at chunk c ^ (row & 15), and row & 15 =
+        // 4 (q & 3) + (lane >> 4)); the lane's address advances by 4 ld
+        // elements per instruction
+        const int r0 = lane >> 4, c = lane & 15;
+        const char* lane_base = base + 8 * ((long)r0 * ld);
+        const long step = 32 * ld;  // bytes per 4 rows
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            const int slot = 64 * q + lane;
-            const int hi = slot >> 4, c = slot & 15;  // row (KC) or k (M/N) of the slot, and its chunk
-            const long off = (long)hi * ld + 2 * (kc ? (c ^ (hi & 15)) : c);
-            __builtin_amdgcn_global_load_lds(base + 8 * off, (lds_void_t*)(img + 1024 * q), 16, 0, 0);
+            const int cg = kc ? (c ^ (4 * (q & 3) + r0)) : c;
+            __builtin_amdgcn_global_load_lds(lane_base + q * step + 16 * cg, (lds_void_t*)(img + 1024 * q), 16, 0, 0);
         }
     } else {
 #pragma unroll
@@ -1298,6 +1333,7 @@ template <int S>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) ozaki_small_full(const OzParams p)
 {
     __shared__ __attribute__((aligned(1024))) char smem[4 * 2 * OZ_IMG];
+    __shared__ __attribute__((aligned(16))) int xch[4 * 32];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long W = (long)gridDim.x * 4;
@@ -1325,7 +1361,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
         if (nxt < p.batch) dma(pwn);
         const uint32_t t = oz_ptr_word(p, after < p.batch ? after : inst, lane);
         uint32_t pwa = 0u;
-        oz_small_instance<S>(p, inst, xa, xb, C, lane, [&] { pwa = oz_settle(t); });
+        oz_small_instance<S>(p, inst, xa, xb, C, lane, xch + 32 * wave, [&] { pwa = oz_settle(t); });
         pw = pwn;
         pwn = pwa;
     }
