@@ -192,13 +192,15 @@ __device__ __forceinline__ void oz_slice16(const double (&x)[16], int e, u32x4 (
             a1 += af;
             a2 = __builtin_fmaf(af, af, a2);
             neg |= (x[i] < 0.0 ? 0xffu : 0u) << (8 * q);
+            // remainders by v_fract_f64 (exact for these non-negative
+            // values below 2^28)
             const double y1 = ldexp(a, 28);
             H[q] = (uint32_t)y1;
-            const double y2 = ldexp(y1 - (double)H[q], 28);
+            const double y2 = ldexp(__builtin_amdgcn_fract(y1), 28);
             L[q] = (uint32_t)y2;
-            const double y3 = ldexp(y2 - (double)L[q], 7);
+            const double y3 = ldexp(__builtin_amdgcn_fract(y2), 7);
             N[q] = (uint32_t)y3;
-            rem |= y3 != (double)N[q];
+            rem |= __builtin_amdgcn_fract(y3) != 0.0;
         }
         uint32_t w[9];
         {
@@ -247,8 +249,12 @@ __global__ void __launch_bounds__(256) ozaki_prep(const OzPrep q)
     const double* X = static_cast<const double*>(q.X[inst]);
     const int nk64 = (q.k + 63) / 64;
 
-    double mx = 0.0;
-    bool bad = false;
+    // Row exponent as exp_of_max gives it, from the largest biased exponent
+    // field E of the row (the high words of |x| compare as the values do):
+    // e = E - 1022 for a normal maximum, OZ_NONFINITE for E = 0x7ff (inf /
+    // nan); a zero or subnormal maximum (E = 0) goes through frexp.
+    uint32_t hmax = 0u;
+    auto hi_abs = [](double v) { return (uint32_t)(__builtin_bit_cast(uint64_t, v) >> 32) & 0x7fffffffu; };
     double xs[NK > 0 ? NK : 1][16];
     if constexpr (NK > 0) {
 #pragma unroll
@@ -259,11 +265,7 @@ __global__ void __launch_bounds__(256) ozaki_prep(const OzPrep q)
         for (int kb = 0; kb < NK; ++kb) {
             if (kb < nk64) {
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    const double a = fabs(xs[kb][i]);
-                    bad |= !isfinite(a);
-                    mx = fmax(mx, a);
-                }
+                for (int i = 0; i < 16; ++i) hmax = max(hmax, hi_abs(xs[kb][i]));
             }
         }
     } else {
@@ -271,19 +273,37 @@ __global__ void __launch_bounds__(256) ozaki_prep(const OzPrep q)
             double x[16];
             oz_load16<KC>(X, q.ld, rr, kb * 64 + 16 * h, q.k, x);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const double a = fabs(x[i]);
-                bad |= !isfinite(a);
-                mx = fmax(mx, a);
-            }
+            for (int i = 0; i < 16; ++i) hmax = max(hmax, hi_abs(x[i]));
         }
     }
 #pragma unroll
-    for (int off = 1; off < 4; off <<= 1) {
-        mx = fmax(mx, __shfl_xor(mx, off));
-        bad |= __shfl_xor((int)bad, off) != 0;
+    for (int off = 1; off < 4; off <<= 1) hmax = max(hmax, (uint32_t)__shfl_xor((int)hmax, off));
+    const int E = (int)(hmax >> 20);
+    int e = E == 0x7ff ? OZ_NONFINITE : E - 1022;
+    if (E == 0) {
+        double mx = 0.0;
+        if constexpr (NK > 0) {
+#pragma unroll
+            for (int kb = 0; kb < NK; ++kb)
+                if (kb < nk64) {
+#pragma unroll
+                    for (int i = 0; i < 16; ++i) mx = fmax(mx, fabs(xs[kb][i]));
+                }
+        } else {
+            for (int kb = 0; kb < nk64; ++kb) {
+                double x[16];
+                oz_load16<KC>(X, q.ld, rr, kb * 64 + 16 * h, q.k, x);
+#pragma unroll
+                for (int i = 0; i < 16; ++i) mx = fmax(mx, fabs(x[i]));
+            }
+        }
+        // the four threads of a row take this branch together (E is the
+        // row's)
+#pragma unroll
+        for (int off = 1; off < 4; off <<= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        e = exp_of_max(mx, false);
     }
-    const int e = row_ok ? exp_of_max(mx, bad) : OZ_NONFINITE;  // padding rows: zero digits
+    if (!row_ok) e = OZ_NONFINITE;  // padding rows: zero digits
 
     float a1 = 0.f, a2 = 0.f;
     uint32_t dsum[9];

@@ -80,7 +80,8 @@ def test_ozaki_fused_subnormal_zero_and_nonfinite_rows(ta, tb, monkeypatch):
     subnormal maximum (the row exponent then comes from frexp, not from the
     biased exponent field), are all zero, mix subnormal and normal values,
     or hold an infinity: the fused kernel gives the general path's result
-    bit for bit (NaN where an infinity reaches)."""
+    bit for bit (NaN where an infinity reaches), and both give the numpy
+    model's, whose row exponents come from numpy's frexp."""
     b = make_batch("f64", ta, tb, 32, 32, 32, 6, alpha=1.0, beta=0.0, seed=61, dist="wide")
     for i in range(6):
         A = oracle.colmajor(b.A[i], *oracle.op_dims(ta, 32, 32), b.lda)
@@ -101,6 +102,10 @@ def test_ozaki_fused_subnormal_zero_and_nonfinite_rows(ta, tb, monkeypatch):
     assert st == 0
     for i, (x, y) in enumerate(zip(fused, general)):
         assert np.array_equal(x, y, equal_nan=True), f"instance {i}"
+        if i != 4:  # and the numpy model (its exponents from numpy's frexp)
+            A, B = op_mats(b, i)
+            want = ozaki_gemm(A, B, 9)
+            assert np.array_equal(oracle.colmajor(y, 32, 32, b.ldc), want), f"instance {i} vs model"
     out = oracle.colmajor(fused[4], 32, 32, b.ldc)
     assert np.isnan(out[7]).all()
 
