@@ -94,7 +94,10 @@ def test_grouped_every_tile_shape(cfg, monkeypatch):
         check(G, dg.results())
 
 
-def test_grouped_edges():
+@pytest.mark.parametrize("cfg", [None, "w32"])
+def test_grouped_edges(cfg, monkeypatch):
+    if cfg is not None:
+        monkeypatch.setenv("SSBLAS_GROUPED_CFG", cfg)
     dims = [(1, 1, 1), (1, 9, 3), (9, 1, 3), (7, 7, 0), (5, 3, 1), (33, 31, 65), (64, 64, 64), (65, 129, 127),
             (128, 128, 128), (129, 2, 300), (2, 129, 300), (512, 512, 512), (3, 500, 8), (500, 3, 8),
             (17, 17, 1000), (1, 1, 1000)]
@@ -109,7 +112,7 @@ def test_grouped_edges():
     check(G, got)
 
 
-@pytest.mark.parametrize("cfg", [None, "g32", "g64", "g128"])
+@pytest.mark.parametrize("cfg", [None, "g32", "g64", "g128", "w32"])
 def test_grouped_matrix_ends(cfg, monkeypatch):
     """Leading dimensions equal to the stored rows (every matrix ends exactly
     at its last element, no padding after it) with tiny k, one-row and
