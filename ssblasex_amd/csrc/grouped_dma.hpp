@@ -1,0 +1,455 @@
+// grouped_dma.hpp -- the grouped kernel's 128x128 workgroup tile for the
+// int8 and fp32 forms with direct-to-LDS buffer loads.
+//
+// Every k-block of A and B moves global -> LDS with buffer_load_dwordx4 ...
+// lds (64 lanes x 16 bytes -> 1 KiB of LDS per wave instruction, no staging
+// registers), S stages deep: while block kt is multiplied, blocks kt+1 ..
+// kt+S-1 are in flight.  One barrier per k-block.  The loads go through
+// buffer descriptors spanning each stored matrix, so they are range-checked
+// like the register-staged tiles' loads: a dword past the end of the matrix
+// lands in LDS as zero, any leading dimension and alignment is read whole.
+//
+// LDS keeps each operand in its storage orientation, in 16-byte chunks
+// (int8: the layout of the batched direct-to-LDS tiles, DmaOperand; fp32
+// K-contiguous: KcOperand; fp32 M/N-contiguous: rows of k, chunks of four
+// M/N values, F32MnOperand below).  Each lane of a DMA instruction writes
+// the chunk slot of its position, so the global chunk a lane fetches is the
+// one the swizzled layout puts at that slot.
+//
+// What the loads cannot get right by themselves, fixed in LDS by the lane
+// that loaded the chunk, after its wave's loads of the block have landed and
+// before the barrier that publishes the block:
+//  * the k tail of a K-contiguous operand (last k-block): elements at k >= K
+//    hold the next row or padding and are cleared (M/N-contiguous chunks at
+//    k >= K start past the end of the matrix and are zero already);
+//  * int8: the dword holding a matrix's last bytes reaches past its end and
+//    reads as zero, so a chunk that crosses the end is read again as the 16
+//    bytes ending at the end and shifted (in a k-block whose footprint
+//    reaches past the end, and only where a lane of the wave holds such a
+//    chunk).
+// Rows and columns outside the instance are not masked: they only feed
+// outputs that are never stored.
+#pragma once
+#include "gemm_dma_kc.hpp"
+
+namespace ssb {
+
+// Elements from index `nvalid` on of a word (WB bytes, E-byte elements) set to 0.
+template <int WB, int E>
+__device__ __forceinline__ void clear_tail(uint32_t* d, int nvalid)
+{
+    const int vb = nvalid * E;
+#pragma unroll
+    for (int i = 0; i < WB / 4; ++i) {
+        const int lo = 4 * i;
+        const uint32_t m = vb >= lo + 4 ? 0xffffffffu : (vb <= lo ? 0u : ((1u << (8 * (vb - lo))) - 1u));
+        d[i] &= m;
+    }
+}
+
+template <int AL>
+struct ElemAligned16;
+template <>
+struct ElemAligned16<4> {
+    typedef uint32_t T __attribute__((ext_vector_type(4), aligned(4)));
+};
+template <>
+struct ElemAligned16<8> {
+    typedef uint32_t T __attribute__((ext_vector_type(4), aligned(8)));
+};
+
+// Fused alpha/beta epilogue of one 16x16 fragment at any ldc and alignment:
+// the lane's 4 rows go out as one (unaligned) vector when all are in range.
+template <class AR>
+__device__ __forceinline__ void gstore_frag(typename AR::Out* C, long ldc, int r0, int c,
+                                            const typename AR::Acc& acc, const Scalars& sc, bool beta0, int m,
+                                            int n)
+{
+    using Out = typename AR::Out;
+    // 16 bytes at element (not vector) alignment
+    using u32x4_e = typename ElemAligned16<sizeof(Out)>::T;
+    constexpr int NV = 4 * (int)sizeof(Out) / 16;
+    if (c >= n) return;
+    Out* dst = C + (long)c * ldc + r0;
+    if (r0 + 4 <= m) {
+        u32x4_e w[NV];
+        Out old[4] = {Out(0), Out(0), Out(0), Out(0)};
+        if (!beta0) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) w[i] = *gbl(reinterpret_cast<const u32x4_e*>(dst) + i);
+            __builtin_memcpy(old, w, sizeof(old));
+        }
+        Out res[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) res[v] = epi(acc[v], old[v], sc, beta0);
+        __builtin_memcpy(w, res, sizeof(res));
+#pragma unroll
+        for (int i = 0; i < NV; ++i) *gbl(reinterpret_cast<u32x4_e*>(dst) + i) = w[i];
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            if (r0 + v < m) {
+                const Out o = beta0 ? Out(0) : *gbl(dst + v);
+                *gbl(dst + v) = epi(acc[v], o, sc, beta0);
+            }
+        }
+    }
+}
+
+// fp32 M/N-contiguous operand in LDS: row = k, 16-byte chunks of 4
+// consecutive M/N values; chunk (row, c) at slot row*CP + (c ^ swz(row)).
+// A fragment lane (r = l%16, g = l/16) needs the four k = 16 ks + 4 g + i,
+// i < 4, of tile row fb + r (the k assignment of KcOperand's chunks), four
+// dword reads from rows that share their swizzle; the swizzle moves the four
+// lane groups' rows (k differing in bits 2..3) to different 64-byte bank
+// ranges.
+template <int MN, int BK>
+struct F32MnOperand {
+    static constexpr int CP = MN / 4;
+    static constexpr int CHUNKS = BK * CP;
+    static constexpr int BYTES = CHUNKS * 16;
+    static_assert(CP >= 16 && BK % 16 == 0, "four 64-byte column groups, whole k-steps");
+
+    __device__ __forceinline__ static int swz(int row) { return ((row >> 2) & 3) << 2; }
+    __device__ __forceinline__ static int slot(int row, int c) { return row * CP + (c ^ swz(row)); }
+    __device__ __forceinline__ static long chunk_off(int row, int c, int mn0, int k0, long ld)
+    {
+        return (long)(k0 + row) * ld + mn0 + 4 * c;
+    }
+    __device__ __forceinline__ static u32x4 frag(const char* lds, int fb, int ks, int lane)
+    {
+        const int mn = fb + (lane & 15), g = lane >> 4;
+        const int c = mn >> 2, w = mn & 3;
+        uint32_t v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = 16 * ks + 4 * g + i;
+            v[i] = *reinterpret_cast<const uint32_t*>(lds + slot(row, c) * 16 + 4 * w);
+        }
+        return u32x4{v[0], v[1], v[2], v[3]};
+    }
+};
+
+// One operand of a grouped DMA tile: the LDS layout for (form, layout), the
+// chunk a slot holds, the k range a K-contiguous chunk covers, and fragment
+// reads for one MFMA k-step.
+template <class AR, bool KC, int MN, int BK>
+struct GDmaOp;
+
+template <bool KC, int MN, int BK>
+struct GDmaOp<ArithI8, KC, MN, BK> {
+    using L = DmaOperand<KC, MN, BK>;
+    static constexpr int CP = L::CP, CHUNKS = L::CHUNKS, BYTES = L::BYTES, EPC = 16;
+    __device__ __forceinline__ static void coords(int s, int& row, int& c)
+    {
+        row = s / CP;
+        c = (s % CP) ^ dma_swz<CP>(row);
+    }
+    __device__ __forceinline__ static long off(int row, int c, int mn0, int k0, long ld)
+    {
+        return L::chunk_off(row, c, mn0, k0, ld);
+    }
+    __device__ __forceinline__ static u32x4 frag(const char* lds, int fb, int ks, int lane)
+    {
+        return L::frag(lds, fb, ks, lane);
+    }
+};
+
+template <int MN, int BK>
+struct GDmaOp<ArithF32, true, MN, BK> {
+    using L = KcOperand<ArithF32, MN, BK>;
+    static constexpr int CP = L::CP, CHUNKS = L::CHUNKS, BYTES = L::BYTES, EPC = 4;
+    __device__ __forceinline__ static void coords(int s, int& row, int& c)
+    {
+        row = s / CP;
+        c = (s % CP) ^ dma_swz<CP>(row);
+    }
+    __device__ __forceinline__ static long off(int row, int c, int mn0, int k0, long ld)
+    {
+        return 4 * L::chunk_off(row, c, mn0, k0, ld);
+    }
+    __device__ __forceinline__ static u32x4 frag(const char* lds, int fb, int ks, int lane)
+    {
+        u32x4 out[1];
+        L::frag(lds, fb + (lane & 15), ks, lane >> 4, out);
+        return out[0];
+    }
+};
+
+template <int MN, int BK>
+struct GDmaOp<ArithF32, false, MN, BK> {
+    using L = F32MnOperand<MN, BK>;
+    static constexpr int CP = L::CP, CHUNKS = L::CHUNKS, BYTES = L::BYTES, EPC = 4;
+    __device__ __forceinline__ static void coords(int s, int& row, int& c)
+    {
+        row = s / CP;
+        c = (s % CP) ^ L::swz(row);
+    }
+    __device__ __forceinline__ static long off(int row, int c, int mn0, int k0, long ld)
+    {
+        return 4 * L::chunk_off(row, c, mn0, k0, ld);
+    }
+    __device__ __forceinline__ static u32x4 frag(const char* lds, int fb, int ks, int lane)
+    {
+        return L::frag(lds, fb, ks, lane);
+    }
+};
+
+// k-depth and stage count of the grouped DMA tile per form: int8 64-k blocks,
+// fp32 16-k blocks (16 KiB per stage either way), three stages (48 KiB: three
+// workgroups per CU).
+template <class AR>
+struct GDmaCfg {
+    static constexpr int BK = AR::ID == ARITH_I8 ? 64 : 16;
+    static constexpr int S = 3;
+};
+
+template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int S>
+struct GDmaTile {
+    using OA = GDmaOp<AR, AKC, BM, BK>;
+    using OB = GDmaOp<AR, BKC, BN, BK>;
+    static constexpr int NWAVE = 4;                      // 2 x 2 waves
+    static constexpr int WTM = BM / 2, WTN = BN / 2;
+    static constexpr int FM = WTM / 16, FN = WTN / 16;
+    static constexpr int STAGE = OA::BYTES + OB::BYTES;
+    static constexpr int LDS_BYTES = S * STAGE;
+    static constexpr int A_PW = OA::CHUNKS / 64 / NWAVE;  // A instructions per wave per stage
+    static constexpr int B_PW = OB::CHUNKS / 64 / NWAVE;
+    static constexpr int PER_WAVE = A_PW + B_PW;
+    static_assert(OA::CHUNKS % (64 * NWAVE) == 0 && OB::CHUNKS % (64 * NWAVE) == 0,
+                  "whole 1 KiB DMA blocks, split evenly over the waves");
+    static_assert(BK % AR::KSTEP == 0 && S >= 2 && S - 2 <= 15, "whole k-steps; stage count");
+    static_assert(PER_WAVE * (S - 2) < 64, "vmcnt range");
+
+    // Byte offset (in the stored matrix) of the chunk that LDS slot s of
+    // operand O holds for the k-block at k0.
+    template <class O>
+    __device__ __forceinline__ static uint32_t src(int s, int mn0, int k0, long ld)
+    {
+        int row, c;
+        O::coords(s, row, c);
+        return (uint32_t)O::off(row, c, mn0, k0, ld);
+    }
+
+    __device__ __forceinline__ static void issue(const TileJob& j, __amdgpu_buffer_rsrc_t ra,
+                                                 __amdgpu_buffer_rsrc_t rb, int kt, char* base, int wave,
+                                                 int lane_in)
+    {
+        const int k0 = kt * BK;
+        const int lane = opaque(lane_in);
+#pragma unroll
+        for (int q = 0; q < A_PW; ++q) {
+            const int inst = wave + q * NWAVE;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(ra, (lds_void_t*)(base + inst * 1024), 16,
+                                                     (int)src<OA>(inst * 64 + lane, j.i0, k0, j.lda), 0, 0, 0);
+        }
+#pragma unroll
+        for (int q = 0; q < B_PW; ++q) {
+            const int inst = wave + q * NWAVE;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rb, (lds_void_t*)(base + OA::BYTES + inst * 1024), 16,
+                                                     (int)src<OB>(inst * 64 + lane, j.j0, k0, j.ldb), 0, 0, 0);
+        }
+    }
+
+    // int8: the chunks this lane loaded for the k-block at k0 that cross the
+    // end of the matrix (`ext` bytes), read again as the 16 bytes ending at
+    // the end and shifted, and written over the loaded chunk.  Only called
+    // for a k-block whose footprint reaches past the end (with a tiny M/N
+    // extent and no padding that can be any k-block, not only the last).
+    template <class O, int PW>
+    __device__ __forceinline__ static void fix_end(char* opbase, __amdgpu_buffer_rsrc_t rs, uint32_t ext, int mn0,
+                                                   int k0, long ld, int wave, int lane)
+    {
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < PW; ++q) {
+            const uint32_t o = src<O>((wave + q * NWAVE) * 64 + lane, mn0, k0, ld);
+            any |= o < ext && o + 16u > ext;
+        }
+        if (!__any(any)) return;
+#pragma unroll
+        for (int q = 0; q < PW; ++q) {
+            const int s = (wave + q * NWAVE) * 64 + lane;
+            const uint32_t o = src<O>(s, mn0, k0, ld);
+            if (o < ext && o + 16u > ext) {
+                uint32_t d[4] = {0u, 0u, 0u, 0u};
+                if (ext < 16u) {
+                    for (uint32_t p = o; p < ext; ++p)
+                        d[(p - o) >> 2] |= __builtin_amdgcn_raw_buffer_load_b8(rs, (int)p, 0, 0) << (8 * ((p - o) & 3));
+                } else {
+                    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(ext - 16u), 0, 0);
+                    const uint32_t sh = o - (ext - 16u), qq = sh >> 2, r = sh & 3;
+                    const uint32_t w[4] = {v[0], v[1], v[2], v[3]};
+                    uint32_t t[5];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const uint32_t a = i + qq;
+                        t[i] = a == 0 ? w[0] : a == 1 ? w[1] : a == 2 ? w[2] : a == 3 ? w[3] : 0u;
+                    }
+                    t[4] = 0u;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) d[i] = __builtin_amdgcn_alignbyte(t[i + 1], t[i], r);
+                }
+                *reinterpret_cast<u32x4*>(opbase + s * 16) = u32x4{d[0], d[1], d[2], d[3]};
+            }
+        }
+    }
+
+    // K-contiguous operand, last k-block (kv < BK valid k from its start):
+    // the elements at k >= kv of the chunks this lane loaded cleared.
+    template <class O, int PW>
+    __device__ __forceinline__ static void clear_k(char* opbase, int kv, int wave, int lane)
+    {
+#pragma unroll
+        for (int q = 0; q < PW; ++q) {
+            const int s = (wave + q * NWAVE) * 64 + lane;
+            int row, c;
+            O::coords(s, row, c);
+            const int valid = kv - c * O::EPC;
+            if (valid < O::EPC) {
+                u32x4* p = reinterpret_cast<u32x4*>(opbase + s * 16);
+                const u32x4 v = *p;
+                uint32_t d[4] = {v[0], v[1], v[2], v[3]};
+                clear_tail<16, AR::E>(d, max(valid, 0));
+                *p = u32x4{d[0], d[1], d[2], d[3]};
+            }
+        }
+    }
+
+    __device__ __forceinline__ static void compute(typename AR::Acc (&acc)[FM][FN], const char* la, int wm, int wn,
+                                                   int lane)
+    {
+        const char* lb = la + OA::BYTES;
+#pragma unroll
+        for (int ks = 0; ks < BK / AR::KSTEP; ++ks) {
+            u32x4 af[FM], bfg[FN];
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm) af[fm] = OA::frag(la, wm * WTM + fm * 16, ks, lane);
+#pragma unroll
+            for (int fn = 0; fn < FN; ++fn) bfg[fn] = OB::frag(lb, wn * WTN + fn * 16, ks, lane);
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm)
+#pragma unroll
+                for (int fn = 0; fn < FN; ++fn) AR::mma(acc[fm][fn], &af[fm], &bfg[fn]);
+        }
+    }
+
+    // Wait until this wave's loads of the block about to be used have landed
+    // (at most `ahead` later blocks still in flight).
+    template <int A = S - 2>
+    __device__ __forceinline__ static void wait_block(int ahead)
+    {
+        if constexpr (A == 0) {
+            wait_vm<0>();
+        } else {
+            if (ahead >= A) wait_vm<A * PER_WAVE>();
+            else wait_block<A - 1>(ahead);
+        }
+    }
+};
+
+// One 128x128 tile of a grouped int8 / fp32 instance; `done_loading` runs
+// once the tile's loads have all landed (the item loop settles the next
+// item's prefetched description there).  Starts with a barrier: the
+// previous item's LDS readers are done.
+template <class AR, bool AKC, bool BKC, class Hook>
+__device__ __forceinline__ void gdma_job(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
+                                         Hook&& done_loading)
+{
+    constexpr int BK = GDmaCfg<AR>::BK, S = GDmaCfg<AR>::S;
+    using T = GDmaTile<AR, AKC, BKC, 128, 128, BK, S>;
+    using Out = typename AR::Out;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 1, wn = wave >> 1;
+    const int nk = (j.k + BK - 1) / BK;
+    const __amdgpu_buffer_rsrc_t ra = make_rsrc(j.A, abytes);
+    const __amdgpu_buffer_rsrc_t rb = make_rsrc(j.B, bbytes);
+    // byte footprint of a k-block (first to last element it reads), to find
+    // the blocks that can hold a chunk crossing the end of a matrix
+    constexpr int E = AR::E;
+    const long sra = AKC ? j.lda : 1, ska = AKC ? 1 : j.lda, srb = BKC ? j.ldb : 1, skb = BKC ? 1 : j.ldb;
+    const uint32_t a_org = (uint32_t)((long)j.i0 * sra * E), b_org = (uint32_t)((long)j.j0 * srb * E);
+    const uint32_t a_kstep = (uint32_t)(BK * ska * E), b_kstep = (uint32_t)(BK * skb * E);
+    const uint32_t a_span = (uint32_t)((127 * sra + (BK - 1) * ska + 1) * E);
+    const uint32_t b_span = (uint32_t)((127 * srb + (BK - 1) * skb + 1) * E);
+
+    typename AR::Acc acc[T::FM][T::FN];
+#pragma unroll
+    for (int fm = 0; fm < T::FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < T::FN; ++fn) acc[fm][fn] = typename AR::Acc{};
+
+    lds_barrier();
+#pragma unroll
+    for (int s = 0; s < S - 1; ++s)
+        if (s < nk) T::issue(j, ra, rb, s, smem + s * T::STAGE, wave, lane);
+    for (int kt = 0; kt < nk; ++kt) {
+        char* cur = smem + (kt % S) * T::STAGE;
+        T::wait_block(min(nk - 1 - kt, S - 2));
+        // fixes in LDS by the lanes that loaded the chunks, before the
+        // barrier that publishes the block
+        const int k0 = kt * BK, kv = j.k - k0;
+        bool fixed = false;
+        if constexpr (AR::E == 1) {
+            if (a_org + (uint32_t)kt * a_kstep + a_span > abytes) {
+                T::template fix_end<typename T::OA, T::A_PW>(cur, ra, abytes, j.i0, k0, j.lda, wave, lane);
+                fixed = true;
+            }
+            if (b_org + (uint32_t)kt * b_kstep + b_span > bbytes) {
+                T::template fix_end<typename T::OB, T::B_PW>(cur + T::OA::BYTES, rb, bbytes, j.j0, k0, j.ldb, wave,
+                                                             lane);
+                fixed = true;
+            }
+        }
+        if (kv < BK) {
+            if constexpr (AKC) T::template clear_k<typename T::OA, T::A_PW>(cur, kv, wave, lane);
+            if constexpr (BKC) T::template clear_k<typename T::OB, T::B_PW>(cur + T::OA::BYTES, kv, wave, lane);
+            fixed = fixed || AKC || BKC;
+        }
+        if (fixed) wait_lgkm0();
+        lds_barrier();
+        if (kt + S - 1 < nk) T::issue(j, ra, rb, kt + S - 1, smem + ((kt + S - 1) % S) * T::STAGE, wave, lane);
+        T::compute(acc, cur, wm, wn, lane);
+    }
+    done_loading();
+
+    Out* C = static_cast<Out*>(j.C);
+    const int row0 = j.i0 + wm * T::WTM + 4 * (lane >> 4), col0 = j.j0 + wn * T::WTN + (lane & 15);
+    if (j.beta0 != 0) {
+#pragma unroll
+        for (int fm = 0; fm < T::FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < T::FN; ++fn)
+                gstore_frag<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, true, j.m, j.n);
+    } else {
+#pragma unroll
+        for (int fm = 0; fm < T::FM; ++fm)
+#pragma unroll
+            for (int fn = 0; fn < T::FN; ++fn)
+                gstore_frag<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, false, j.m, j.n);
+    }
+}
+
+template <class AR>
+constexpr int gdma_lds_bytes()
+{
+    using T = GDmaTile<AR, true, true, 128, 128, GDmaCfg<AR>::BK, GDmaCfg<AR>::S>;
+    using U = GDmaTile<AR, false, false, 128, 128, GDmaCfg<AR>::BK, GDmaCfg<AR>::S>;
+    return T::LDS_BYTES > U::LDS_BYTES ? T::LDS_BYTES : U::LDS_BYTES;
+}
+
+template <class AR, class Hook>
+__device__ __forceinline__ void g_run_dma(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
+                                          Hook&& done_loading)
+{
+    if (j.akc) {
+        if (j.bkc) gdma_job<AR, true, true>(j, abytes, bbytes, smem, done_loading);
+        else gdma_job<AR, true, false>(j, abytes, bbytes, smem, done_loading);
+    } else {
+        if (j.bkc) gdma_job<AR, false, true>(j, abytes, bbytes, smem, done_loading);
+        else gdma_job<AR, false, false>(j, abytes, bbytes, smem, done_loading);
+    }
+}
+
+}  // namespace ssb

@@ -102,6 +102,29 @@ int forced_cfg(long m, long n)
     return -1;
 }
 
+// Shape of the strip that covers `r` rows (columns) left over past the last
+// whole tile of shape `cfg`: the smallest shape at least r wide (one-wave
+// 32x32 items for r <= 32, where the form has them), `cfg` itself when none
+// is smaller.  int8 keeps whole tiles: its tiles are bound by memory, not by
+// the MFMA work of the rows past the edge, and the one-wave items, which
+// have one k-block in flight, would be slower.
+int edge_cfg(int cfg, int form, int r)
+{
+    if (r <= 0 || form == K_I8) return cfg;
+    const bool waves = form == K_I8 || form == K_F32;
+    if (r <= 32 && tile_of(cfg) > 32) return waves ? GCFG_W32 : GCFG_32;
+    if (r <= 64 && tile_of(cfg) > 64) return GCFG_64;
+    return cfg;
+}
+
+// SSBLAS_GROUPED_EDGES=0 keeps whole tiles of one shape over every instance
+// (tuning and test runs compare the two).
+bool split_edges()
+{
+    const char* e = getenv("SSBLAS_GROUPED_EDGES");
+    return e == nullptr || strcmp(e, "0") != 0;
+}
+
 int pick_cfg(int form, long m, long n, long k, bool beta0)
 {
     int top = grouped_max_cfg(form);
@@ -287,32 +310,54 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
         if (I.m == 0 || I.n == 0 || I.k == 0) continue;
         const HostGroup& G = groups[(size_t)I.group];
         const int cfg = pick_cfg(G.form, I.m, I.n, I.k, G.beta0 != 0);
-        const int t = tile_of(cfg);
-        const double ns = item_ns(G.form, cfg, I.k, G.beta0 != 0);
         const int f = G.form == K_F64 ? GF_F64 : other_family;
         const size_t begin = tiles.size();
-        for (int j0 = 0; j0 < I.n; j0 += t)
-            for (int i0 = 0; i0 < I.m; i0 += t) {
-                GRec r;
-                r.inst = (int32_t)i;
-                r.i0 = i0;
-                r.j0 = j0;
-                r.bits = grec_bits(cfg, G.form, G.akc, G.bkc, G.beta0);
-                r.m = I.m;
-                r.n = I.n;
-                r.k = I.k;
-                r.lda = (int32_t)I.lda;  // below 2^31: the extent fits 31 bits
-                r.ldb = (int32_t)I.ldb;
-                r.ldc = (int32_t)I.ldc;
-                r.abytes = I.abytes;
-                r.bbytes = I.bbytes;
-                memcpy(r.alpha, G.alpha, sizeof(r.alpha));
-                memcpy(r.beta, G.beta, sizeof(r.beta));
-                const HostItem h{r, ns};
-                if (cfg == GCFG_W32) wlists[f].push_back(h);
-                else tiles.push_back(h);
+        double inst_ns = 0.0;
+        auto emit = [&](int c, int i0, int j0) {
+            GRec r;
+            r.inst = (int32_t)i;
+            r.i0 = i0;
+            r.j0 = j0;
+            r.bits = grec_bits(c, G.form, G.akc, G.bkc, G.beta0);
+            r.m = I.m;
+            r.n = I.n;
+            r.k = I.k;
+            r.lda = (int32_t)I.lda;  // below 2^31: the extent fits 31 bits
+            r.ldb = (int32_t)I.ldb;
+            r.ldc = (int32_t)I.ldc;
+            r.abytes = I.abytes;
+            r.bbytes = I.bbytes;
+            memcpy(r.alpha, G.alpha, sizeof(r.alpha));
+            memcpy(r.beta, G.beta, sizeof(r.beta));
+            const HostItem h{r, item_ns(G.form, c, I.k, G.beta0 != 0)};
+            if (c == GCFG_W32) {
+                wlists[f].push_back(h);
+            } else {
+                tiles.push_back(h);
+                inst_ns += h.ns;
             }
-        if (tiles.size() > begin) work.push_back(InstWork{f, ns * (double)(tiles.size() - begin), begin, tiles.size()});
+        };
+        // Whole tiles of the chosen shape, then the rows (columns) left over
+        // past the last whole tile in strips of the smallest shape that
+        // covers them: a 128-tile over 22 leftover rows would spend its
+        // whole k-loop on 106 rows that are never stored.
+        const int t = tile_of(cfg);
+        const int mf = split_edges() ? I.m / t * t : I.m, nf = split_edges() ? I.n / t * t : I.n;
+        const int em = edge_cfg(cfg, G.form, I.m - mf), en = edge_cfg(cfg, G.form, I.n - nf);
+        const int tm = tile_of(em), tn = tile_of(en);
+        for (int j0 = 0; j0 < nf; j0 += t)
+            for (int i0 = 0; i0 < mf; i0 += t) emit(cfg, i0, j0);
+        if (mf < I.m)  // bottom strip: leftover rows under the whole tiles' columns
+            for (int j0 = 0; j0 < nf; j0 += tm) emit(em, mf, j0);
+        if (nf < I.n)  // right strip: leftover columns beside the whole tiles' rows
+            for (int i0 = 0; i0 < mf; i0 += tn) emit(en, i0, nf);
+        if (mf < I.m && nf < I.n) {  // corner
+            const int ec = tile_of(em) >= tile_of(en) ? em : en;
+            const int tc = tile_of(ec);
+            for (int j0 = nf; j0 < I.n; j0 += tc)
+                for (int i0 = mf; i0 < I.m; i0 += tc) emit(ec, i0, j0);
+        }
+        if (tiles.size() > begin) work.push_back(InstWork{f, inst_ns, begin, tiles.size()});
     }
     std::vector<size_t> order(work.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;

@@ -23,23 +23,11 @@
 
 #include "gemm_kernels.hpp"
 #include "grouped.hpp"
+#include "grouped_dma.hpp"
 #include "launch.hpp"
 #include "occupancy.hpp"
 
 namespace ssb {
-
-// Elements from index `nvalid` on of a word (WB bytes, E-byte elements) set to 0.
-template <int WB, int E>
-__device__ __forceinline__ void clear_tail(uint32_t* d, int nvalid)
-{
-    const int vb = nvalid * E;
-#pragma unroll
-    for (int i = 0; i < WB / 4; ++i) {
-        const int lo = 4 * i;
-        const uint32_t m = vb >= lo + 4 ? 0xffffffffu : (vb <= lo ? 0u : ((1u << (8 * (vb - lo))) - 1u));
-        d[i] &= m;
-    }
-}
 
 // Buffer-load staging of one operand tile (ROWS x BK), same unit and word
 // layout as StagerImpl<KC>, so StagerImpl::store writes it to LDS.
@@ -216,55 +204,6 @@ struct GStage {
         else LN::I::store(raw, lds, plane_bytes, rowb, tid);
     }
 };
-
-template <int AL>
-struct ElemAligned16;
-template <>
-struct ElemAligned16<4> {
-    typedef uint32_t T __attribute__((ext_vector_type(4), aligned(4)));
-};
-template <>
-struct ElemAligned16<8> {
-    typedef uint32_t T __attribute__((ext_vector_type(4), aligned(8)));
-};
-
-// Fused alpha/beta epilogue of one 16x16 fragment at any ldc and alignment:
-// the lane's 4 rows go out as one (unaligned) vector when all are in range.
-template <class AR>
-__device__ __forceinline__ void gstore_frag(typename AR::Out* C, long ldc, int r0, int c,
-                                            const typename AR::Acc& acc, const Scalars& sc, bool beta0, int m,
-                                            int n)
-{
-    using Out = typename AR::Out;
-    // 16 bytes at element (not vector) alignment
-    using u32x4_e = typename ElemAligned16<sizeof(Out)>::T;
-    constexpr int NV = 4 * (int)sizeof(Out) / 16;
-    if (c >= n) return;
-    Out* dst = C + (long)c * ldc + r0;
-    if (r0 + 4 <= m) {
-        u32x4_e w[NV];
-        Out old[4] = {Out(0), Out(0), Out(0), Out(0)};
-        if (!beta0) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) w[i] = *gbl(reinterpret_cast<const u32x4_e*>(dst) + i);
-            __builtin_memcpy(old, w, sizeof(old));
-        }
-        Out res[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) res[v] = epi(acc[v], old[v], sc, beta0);
-        __builtin_memcpy(w, res, sizeof(res));
-#pragma unroll
-        for (int i = 0; i < NV; ++i) *gbl(reinterpret_cast<u32x4_e*>(dst) + i) = w[i];
-    } else {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            if (r0 + v < m) {
-                const Out o = beta0 ? Out(0) : *gbl(dst + v);
-                *gbl(dst + v) = epi(acc[v], o, sc, beta0);
-            }
-        }
-    }
-}
 
 typedef int32_t gi32x2 __attribute__((ext_vector_type(2)));
 typedef __attribute__((address_space(3))) gi32x2 g_lds_i32x2;
@@ -574,10 +513,19 @@ struct GShape {
     static constexpr int LDS = TileShape<AR, true, true, BM, BN, BK, 2, 2>::LDS_BYTES;
 };
 
+// LDS of the 128x128 tile: direct-to-LDS stages for int8 and fp32, the
+// register-staged double buffer otherwise.
+template <class AR>
+constexpr int g128_lds()
+{
+    if constexpr (AR::ID == ARITH_I8 || AR::ID == ARITH_F32) return gdma_lds_bytes<AR>();
+    else return GShape<AR, GCFG_128>::LDS;
+}
+
 template <class AR>
 constexpr int g_lds_max()
 {
-    constexpr int a = GShape<AR, GCFG_32>::LDS, b = GShape<AR, GCFG_64>::LDS, c = GShape<AR, GCFG_128>::LDS;
+    constexpr int a = GShape<AR, GCFG_32>::LDS, b = GShape<AR, GCFG_64>::LDS, c = g128_lds<AR>();
     constexpr int ab = a > b ? a : b;
     return grouped_max_cfg(AR::ID) == GCFG_128 ? (ab > c ? ab : c) : ab;
 }
@@ -593,8 +541,12 @@ __device__ __forceinline__ void g_run(int cfg, const TileJob& j, uint32_t abytes
     }
     if constexpr (grouped_max_cfg(AR::ID) == GCFG_128) {
         if (cfg == GCFG_128) {
-            using G = GShape<AR, GCFG_128>;
-            gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, first_block);
+            if constexpr (AR::ID == ARITH_I8 || AR::ID == ARITH_F32) {
+                g_run_dma<AR>(j, abytes, bbytes, smem, first_block);
+            } else {
+                using G = GShape<AR, GCFG_128>;
+                gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, first_block);
+            }
             return;
         }
     }
@@ -724,6 +676,14 @@ __device__ __forceinline__ void g_next(const GroupedDevice& d, GCursor& c)
     g_skip(d, c);
 }
 
+// Waves per SIMD the family's kernel is compiled for: three workgroups per
+// CU (48 KiB of LDS each) where the registers allow; the bf16 split forms
+// hold two or three operand planes per fragment and keep two.
+constexpr int g_waves_per_eu(int second_form)
+{
+    return second_form == ARITH_BF16X3 || second_form == ARITH_BF16X6 ? 2 : 3;
+}
+
 // FA: the family's first form; FB: its second (== FA for one-form families).
 // Workgroup items first (the longer ones): workgroup b, on XCD b % 8, takes
 // its fixed share of queue b % GQ (the queue is sorted longest first, so
@@ -737,7 +697,7 @@ __device__ __forceinline__ void g_next(const GroupedDevice& d, GCursor& c)
 // item starts with its description at hand instead of a chain of dependent
 // loads, each in order behind the previous item's stores.
 template <class FA, class FB>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) grouped_gemm(const GroupedDevice d)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(g_waves_per_eu(FB::ID)))) grouped_gemm(const GroupedDevice d)
 {
     constexpr int LA = g_lds_max<FA>(), LB = g_lds_max<FB>();
     __shared__ __attribute__((aligned(16))) char smem[LA > LB ? LA : LB];
@@ -771,6 +731,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) g
         }
     }
     if (d.nwitems > 0) {
+        // the workgroup's last tile may still be read from LDS by another wave
+        lds_barrier();
         const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
         const int nw = 4 * (int)gridDim.x;
         const int w0 = 4 * (int)blockIdx.x + wave;

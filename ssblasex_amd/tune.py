@@ -18,8 +18,9 @@ library falls back to its size heuristic for buckets the table lacks.
 
 --set grouped tunes the tile shape of grouped plans (g32, g64, g128, w32) per
 (form, size class of m, n, k), size classes <= 32, <= 64, <= 128, larger:
-for every bucket it builds a plan over a few hundred seeded instances drawn
-inside the bucket (random layouts, padded leading dimensions), times its
+for every bucket it builds a plan over seeded instances drawn inside the
+bucket (random layouts, padded leading dimensions; enough of them to fill the
+GPU several times over, so that the time is the class's throughput), times its
 execution with each shape forced (SSBLAS_GROUPED_CFG) and records the best
 as a row "form 0 0 cm cn ck 9 gNN us".  Rows of other sets already in the
 table are kept.
@@ -172,7 +173,10 @@ def tune_grouped(S, torch, forms, quick, log):
                 for ck in range(4):
                     lo_hi = [CLASS_RANGE[c] for c in (cm, cn, ck)]
                     mean = [_m.sqrt(a * b) for a, b in lo_hi]
-                    count = int(min(2000, max(64, 4e9 / (2 * mean[0] * mean[1] * mean[2]))))
+                    # enough work (~4e10 flops, at most 8000 instances) to fill the GPU several
+                    # times over, so that the time measures throughput, which is what the
+                    # class's share of a large mixed plan costs, not one plan's tail
+                    count = int(min(8000, max(256, 4e10 / (2 * mean[0] * mean[1] * mean[2]))))
                     if quick:
                         count = min(count, 256)
                     dims = [[int(_m.floor(_m.exp(rng.uniform(_m.log(a), _m.log(b + 1))))) for a, b in lo_hi]
@@ -249,6 +253,7 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--forms", default="i8,f32,f64", help="forms of --set grouped (comma-separated)")
     a = ap.parse_args(argv)
     import torch
 
@@ -265,7 +270,7 @@ def main(argv=None):
         time_config(S, torch, "i8", 1, 1, 512, 512, 512, 64, "tile128", 50, 0)
     t0 = time.time()
     if a.set == "grouped":
-        rows.update(tune_grouped(S, torch, ["i8", "f32", "f64"], a.quick, lambda msg: print(msg, flush=True)))
+        rows.update(tune_grouped(S, torch, a.forms.split(","), a.quick, lambda msg: print(msg, flush=True)))
     for (form, ta, tb, m, n, k, batch) in (problem_sets(a.set) if a.set != "grouped" else []):
         res = {}
         for cfg in candidates(form, m, n):

@@ -81,11 +81,17 @@ def test_grouped_beta_zero_ignores_nan():
     check(G, got)
 
 
-@pytest.mark.parametrize("cfg", ["g32", "g64", "g128", "w32", "w32:g64", "w32:g128", "g32:w32"])
+@pytest.mark.parametrize("cfg", ["g32", "g64", "g128", "w32", "w32:g64", "w32:g128", "g32:w32", "g64/whole",
+                                 "g128/whole"])
 def test_grouped_every_tile_shape(cfg, monkeypatch):
     """Each tile shape forced (w32: one wave per 32x32 block, taken by the
     waves one at a time after the workgroup tiles), and queues that mix
-    wave items and workgroup items ("small:rest")."""
+    wave items and workgroup items ("small:rest"); "/whole": the shape over
+    the whole instance, partial tiles at its edges instead of strips of a
+    smaller shape."""
+    if cfg.endswith("/whole"):
+        cfg = cfg[:-6]
+        monkeypatch.setenv("SSBLAS_GROUPED_EDGES", "0")
     monkeypatch.setenv("SSBLAS_GROUPED_CFG", cfg)
     G = make_grouped(48, seed=0x55B1A5 + 53, hi=300, group=8)
     dg = DeviceGrouped(G)
@@ -112,19 +118,51 @@ def test_grouped_edges(cfg, monkeypatch):
     check(G, got)
 
 
-@pytest.mark.parametrize("cfg", [None, "g32", "g64", "g128", "w32"])
+@pytest.mark.parametrize("cfg", [None, "g32", "g64", "g128", "w32", "g64/whole", "g128/whole"])
 def test_grouped_matrix_ends(cfg, monkeypatch):
     """Leading dimensions equal to the stored rows (every matrix ends exactly
     at its last element, no padding after it) with tiny k, one-row and
     one-column operands: the word that holds an int8 matrix's last bytes
     falls in an earlier row tile or an earlier k-block than the last one,
     and k tails of K-contiguous operands read the next row's data."""
+    if cfg is not None and cfg.endswith("/whole"):
+        cfg = cfg[:-6]
+        monkeypatch.setenv("SSBLAS_GROUPED_EDGES", "0")
     if cfg is not None:
         monkeypatch.setenv("SSBLAS_GROUPED_CFG", cfg)
     dims = [(33, 5, 1), (33, 33, 2), (1, 3, 65), (1, 1, 129), (2, 70, 65), (3, 3, 3), (31, 2, 17), (65, 1, 1),
             (129, 130, 1), (2, 2, 64), (3, 1, 128), (70, 3, 2), (1, 65, 63), (5, 5, 129)]
     dims = dims * 4
     G = make_grouped(len(dims), seed=0x55B1A5 + 57, group=1, dims=dims, pad=0, types=("i8", "f32"))
+    dg = DeviceGrouped(G)
+    with plan_for(dg) as p:
+        assert p.execute() == 0
+        check(G, dg.results())
+
+
+@pytest.mark.parametrize("cfg", [None, "g128", "g64", "w32", "g128/whole"])
+def test_grouped_nan_padding(cfg, monkeypatch):
+    """fp32 operands whose storage outside the logical matrix (rows between
+    the stored rows and the leading dimension) holds NaN, every layout pair,
+    k tails of every length: the elements past K that a K-contiguous k-block
+    reads, and the rows and columns outside the instance, must not reach a
+    stored result."""
+    if cfg is not None and cfg.endswith("/whole"):
+        cfg = cfg[:-6]
+        monkeypatch.setenv("SSBLAS_GROUPED_EDGES", "0")
+    if cfg is not None:
+        monkeypatch.setenv("SSBLAS_GROUPED_CFG", cfg)
+    dims = [(130, 129, 17), (128, 128, 1), (200, 140, 33), (129, 3, 15), (3, 129, 47), (140, 200, 100),
+            (257, 130, 130), (128, 128, 16), (160, 150, 63), (131, 133, 65)]
+    dims = dims * 4
+    G = make_grouped(len(dims), seed=0x55B1A5 + 58, group=1, dims=dims, types=("f32",), pad=16)
+    for b in G.inst:
+        ra, ca = oracle.op_dims(b.transa, b.m, b.k)
+        rb, cb = oracle.op_dims(b.transb, b.k, b.n)
+        for X, r, c, ld in ((b.A[0], ra, ca, b.lda), (b.B[0], rb, cb, b.ldb)):
+            mask = np.ones_like(X, dtype=bool)
+            oracle.colmajor(mask, r, c, ld)[:] = False
+            X[mask] = np.nan
     dg = DeviceGrouped(G)
     with plan_for(dg) as p:
         assert p.execute() == 0
