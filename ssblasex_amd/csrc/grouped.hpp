@@ -54,17 +54,28 @@ constexpr uint32_t grec_bits(int cfg, int form, int akc, int bkc, int beta0)
 // Workgroup items are split into GQ queues, one per XCD: every tile of an
 // instance is in one queue, so the instance's operands are fetched into one
 // XCD's L2 (workgroups are dispatched to the eight XCDs round-robin, so
-// workgroup b runs on XCD b % 8 and takes its share of queue b % GQ).
+// workgroup b runs on XCD b % 8 and claims from queue b % GQ first).
 constexpr int GQ = 8;
 
+// Claim counters of a family's launch (GroupedDevice::ctrs), each in its
+// own 256-byte line: one per queue, one counting the workgroups that have
+// finished.  Zero when a launch starts; the last workgroup to finish zeroes
+// them again.
+constexpr int GCTR_STRIDE = 64;  // ints between counters
+constexpr int GCTR_DONE = GQ, GCTR_COUNT = GQ + 1;
+
 struct GroupedDevice {
-    const GRec* recs;    // workgroup items queue by queue, then wave items; each sorted by estimated cost, longest first
-    int nitems;          // workgroup items (all queues)
-    int nwitems;         // wave items (recs[nitems .. nitems + nwitems))
-    int qoff[GQ + 1];    // queue q holds recs[qoff[q] .. qoff[q + 1])
+    // Entries queue by queue, each queue sorted by estimated cost, longest
+    // first.  Entry e is records 4 e .. 4 e + 3, one per wave of the
+    // workgroup that takes it: a tile (four copies of its record) or a
+    // bundle of four one-wave items (empty ones have m = n = k = 0).
+    const GRec* recs;
+    int nitems;          // entries (all queues)
+    int qoff[GQ + 1];    // queue q holds entries qoff[q] .. qoff[q + 1] - 1
     const void* const* A;  // the caller's device arrays of device pointers
     const void* const* B;
     void* const* C;
+    int* ctrs;           // claim counters, GCTR_STRIDE ints apart
 };
 
 // k-depth of a grouped tile per form and shape.

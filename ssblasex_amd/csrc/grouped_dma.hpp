@@ -1,5 +1,5 @@
-// grouped_dma.hpp -- the grouped kernel's 128x128 workgroup tile for the
-// int8 and fp32 forms with direct-to-LDS buffer loads.
+// grouped_dma.hpp -- the grouped kernel's 64x64 and 128x128 workgroup tiles
+// for the int8 and fp32 forms with direct-to-LDS buffer loads.
 //
 // Every k-block of A and B moves global -> LDS with buffer_load_dwordx4 ...
 // lds (64 lanes x 16 bytes -> 1 KiB of LDS per wave instruction, no staging
@@ -195,12 +195,12 @@ struct GDmaOp<ArithF32, false, MN, BK> {
     }
 };
 
-// k-depth and stage count of the grouped DMA tile per form: int8 64-k blocks,
-// fp32 16-k blocks (16 KiB per stage either way), three stages (48 KiB: three
-// workgroups per CU).
-template <class AR>
+// k-depth and stage count of a grouped DMA tile per form and tile size:
+// 16 KiB per stage (int8 128x64 / 64x128 k-blocks, fp32 128x16 / 64x32),
+// three stages (48 KiB: three workgroups per CU).
+template <class AR, int T>
 struct GDmaCfg {
-    static constexpr int BK = AR::ID == ARITH_I8 ? 64 : 16;
+    static constexpr int BK = (AR::ID == ARITH_I8 ? 64 : 16) * (128 / T);
     static constexpr int S = 3;
 };
 
@@ -348,16 +348,16 @@ struct GDmaTile {
     }
 };
 
-// One 128x128 tile of a grouped int8 / fp32 instance; `done_loading` runs
+// One TxT tile (T = 64 or 128) of a grouped int8 / fp32 instance; `done_loading` runs
 // once the tile's loads have all landed (the item loop settles the next
 // item's prefetched description there).  Starts with a barrier: the
 // previous item's LDS readers are done.
-template <class AR, bool AKC, bool BKC, class Hook>
+template <class AR, int TS, bool AKC, bool BKC, class Hook>
 __device__ __forceinline__ void gdma_job(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
                                          Hook&& done_loading)
 {
-    constexpr int BK = GDmaCfg<AR>::BK, S = GDmaCfg<AR>::S;
-    using T = GDmaTile<AR, AKC, BKC, 128, 128, BK, S>;
+    constexpr int BK = GDmaCfg<AR, TS>::BK, S = GDmaCfg<AR, TS>::S;
+    using T = GDmaTile<AR, AKC, BKC, TS, TS, BK, S>;
     using Out = typename AR::Out;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -371,8 +371,8 @@ __device__ __forceinline__ void gdma_job(const TileJob& j, uint32_t abytes, uint
     const long sra = AKC ? j.lda : 1, ska = AKC ? 1 : j.lda, srb = BKC ? j.ldb : 1, skb = BKC ? 1 : j.ldb;
     const uint32_t a_org = (uint32_t)((long)j.i0 * sra * E), b_org = (uint32_t)((long)j.j0 * srb * E);
     const uint32_t a_kstep = (uint32_t)(BK * ska * E), b_kstep = (uint32_t)(BK * skb * E);
-    const uint32_t a_span = (uint32_t)((127 * sra + (BK - 1) * ska + 1) * E);
-    const uint32_t b_span = (uint32_t)((127 * srb + (BK - 1) * skb + 1) * E);
+    const uint32_t a_span = (uint32_t)(((TS - 1) * sra + (BK - 1) * ska + 1) * E);
+    const uint32_t b_span = (uint32_t)(((TS - 1) * srb + (BK - 1) * skb + 1) * E);
 
     typename AR::Acc acc[T::FM][T::FN];
 #pragma unroll
@@ -431,24 +431,24 @@ __device__ __forceinline__ void gdma_job(const TileJob& j, uint32_t abytes, uint
     }
 }
 
-template <class AR>
+template <class AR, int TS>
 constexpr int gdma_lds_bytes()
 {
-    using T = GDmaTile<AR, true, true, 128, 128, GDmaCfg<AR>::BK, GDmaCfg<AR>::S>;
-    using U = GDmaTile<AR, false, false, 128, 128, GDmaCfg<AR>::BK, GDmaCfg<AR>::S>;
+    using T = GDmaTile<AR, true, true, TS, TS, GDmaCfg<AR, TS>::BK, GDmaCfg<AR, TS>::S>;
+    using U = GDmaTile<AR, false, false, TS, TS, GDmaCfg<AR, TS>::BK, GDmaCfg<AR, TS>::S>;
     return T::LDS_BYTES > U::LDS_BYTES ? T::LDS_BYTES : U::LDS_BYTES;
 }
 
-template <class AR, class Hook>
+template <class AR, int TS, class Hook>
 __device__ __forceinline__ void g_run_dma(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
                                           Hook&& done_loading)
 {
     if (j.akc) {
-        if (j.bkc) gdma_job<AR, true, true>(j, abytes, bbytes, smem, done_loading);
-        else gdma_job<AR, true, false>(j, abytes, bbytes, smem, done_loading);
+        if (j.bkc) gdma_job<AR, TS, true, true>(j, abytes, bbytes, smem, done_loading);
+        else gdma_job<AR, TS, true, false>(j, abytes, bbytes, smem, done_loading);
     } else {
-        if (j.bkc) gdma_job<AR, false, true>(j, abytes, bbytes, smem, done_loading);
-        else gdma_job<AR, false, false>(j, abytes, bbytes, smem, done_loading);
+        if (j.bkc) gdma_job<AR, TS, false, true>(j, abytes, bbytes, smem, done_loading);
+        else gdma_job<AR, TS, false, false>(j, abytes, bbytes, smem, done_loading);
     }
 }
 

@@ -292,18 +292,19 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
     }
     if (A == nullptr || B == nullptr || C == nullptr) return SSBLAS_STATUS_INVALID_VALUE;
 
-    // Work lists per family (workgroup items, wave items), longest first.
+    // Work lists per family, longest first.  Every tile of an instance, and
+    // its one-wave items, go to one of the GQ XCD queues (instances longest
+    // first onto the queue with the least estimated work so far).  A queue's
+    // entries are its tiles and bundles of four one-wave items of similar
+    // cost (one item per wave of the workgroup that claims the bundle), so
+    // the small items run beside the tiles instead of after them.
     const int other_family = fp32_form == K_BF16X3 ? GF_I8_BF16X3 : fp32_form == K_BF16X6 ? GF_I8_BF16X6 : GF_I8_F32;
-    // Workgroup tiles go to the XCD queues instance by instance (all tiles of
-    // an instance in one queue), instances longest first onto the queue with
-    // the least estimated work so far; wave items stay in one list.
-    std::vector<HostItem> qlists[GF_COUNT][GQ], wlists[GF_COUNT];
     struct InstWork {
         int f;
         double ns;
-        size_t begin, end;  // its tiles in `tiles`
+        size_t begin, end;  // its items in `items`
     };
-    std::vector<HostItem> tiles;
+    std::vector<HostItem> items;
     std::vector<InstWork> work;
     for (long i = 0; i < total; ++i) {
         const HostInst& I = insts[(size_t)i];
@@ -311,7 +312,7 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
         const HostGroup& G = groups[(size_t)I.group];
         const int cfg = pick_cfg(G.form, I.m, I.n, I.k, G.beta0 != 0);
         const int f = G.form == K_F64 ? GF_F64 : other_family;
-        const size_t begin = tiles.size();
+        const size_t begin = items.size();
         double inst_ns = 0.0;
         auto emit = [&](int c, int i0, int j0) {
             GRec r;
@@ -330,12 +331,8 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
             memcpy(r.alpha, G.alpha, sizeof(r.alpha));
             memcpy(r.beta, G.beta, sizeof(r.beta));
             const HostItem h{r, item_ns(G.form, c, I.k, G.beta0 != 0)};
-            if (c == GCFG_W32) {
-                wlists[f].push_back(h);
-            } else {
-                tiles.push_back(h);
-                inst_ns += h.ns;
-            }
+            items.push_back(h);
+            inst_ns += c == GCFG_W32 ? h.ns / 4 : h.ns;  // four one-wave items run at once
         };
         // Whole tiles of the chosen shape, then the rows (columns) left over
         // past the last whole tile in strips of the smallest shape that
@@ -357,36 +354,73 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
             for (int j0 = nf; j0 < I.n; j0 += tc)
                 for (int i0 = mf; i0 < I.m; i0 += tc) emit(ec, i0, j0);
         }
-        if (tiles.size() > begin) work.push_back(InstWork{f, inst_ns, begin, tiles.size()});
+        if (items.size() > begin) work.push_back(InstWork{f, inst_ns, begin, items.size()});
     }
     std::vector<size_t> order(work.size());
     for (size_t i = 0; i < order.size(); ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work[a].ns > work[b].ns; });
     double load[GF_COUNT][GQ] = {};
+    std::vector<HostItem> qitems[GF_COUNT][GQ];
     for (size_t w : order) {
         const InstWork& W = work[w];
         int best = 0;
         for (int qi = 1; qi < GQ; ++qi)
             if (load[W.f][qi] < load[W.f][best]) best = qi;
         load[W.f][best] += W.ns;
-        for (size_t x = W.begin; x < W.end; ++x) qlists[W.f][best].push_back(tiles[x]);
+        for (size_t x = W.begin; x < W.end; ++x) qitems[W.f][best].push_back(items[x]);
     }
 
-    long nitems = 0;
+    // Entries: a tile's record four times (each wave reads record 4 e + its
+    // index), or a bundle of four one-wave items (a bundle short of four is
+    // filled with empty items, m = n = k = 0).
+    struct HostEntry {
+        GRec rec[4];
+        double ns;
+    };
+    std::vector<HostEntry> qlists[GF_COUNT][GQ];
     auto by_cost = [](const HostItem& a, const HostItem& b) { return a.ns > b.ns; };
+    long nitems = 0;
     for (int f = 0; f < GF_COUNT; ++f) {
         for (int qi = 0; qi < GQ; ++qi) {
-            std::stable_sort(qlists[f][qi].begin(), qlists[f][qi].end(), by_cost);
-            nitems += (long)qlists[f][qi].size();
+            std::vector<HostItem>& Q = qitems[f][qi];
+            std::stable_sort(Q.begin(), Q.end(), by_cost);
+            std::vector<HostEntry>& E = qlists[f][qi];
+            std::vector<const HostItem*> waves;
+            for (const HostItem& h : Q) {
+                if ((h.rec.bits & 3u) == (uint32_t)GCFG_W32) {
+                    waves.push_back(&h);
+                } else {
+                    HostEntry e;
+                    for (int w = 0; w < 4; ++w) e.rec[w] = h.rec;
+                    e.ns = h.ns;
+                    E.push_back(e);
+                }
+            }
+            for (size_t x = 0; x < waves.size(); x += 4) {
+                HostEntry e;
+                e.ns = waves[x]->ns;  // the longest of the four
+                for (int w = 0; w < 4; ++w) {
+                    if (x + (size_t)w < waves.size()) {
+                        e.rec[w] = waves[x + (size_t)w]->rec;
+                    } else {
+                        e.rec[w] = waves[x]->rec;
+                        e.rec[w].m = e.rec[w].n = e.rec[w].k = 0;
+                    }
+                }
+                E.push_back(e);
+            }
+            std::stable_sort(E.begin(), E.end(), [](const HostEntry& a, const HostEntry& b) { return a.ns > b.ns; });
+            nitems += (long)E.size();
         }
-        std::stable_sort(wlists[f].begin(), wlists[f].end(), by_cost);
-        nitems += (long)wlists[f].size();
     }
     if (nitems > INT_MAX) return SSBLAS_STATUS_NOTIMPLEMENTED_ERROR;
 
     // Device image: the records of each family
     const size_t off_items = 0;
-    const size_t bytes = (size_t)std::max(nitems, 1L) * sizeof(GRec);
+    // then each family's claim counters (zero: the kernel re-arms them itself)
+    const size_t off_ctrs = (size_t)std::max(nitems, 1L) * 4 * sizeof(GRec);
+    constexpr size_t CTR_BYTES = (size_t)GCTR_COUNT * GCTR_STRIDE * sizeof(int);
+    const size_t bytes = off_ctrs + GF_COUNT * CTR_BYTES;
     // the plan's device image is library-owned workspace
     if (bytes > workspace_limit()) return SSBLAS_STATUS_ALLOC_FAILED;
     if (!device_accessible(A) || !device_accessible(B) || !device_accessible(C))
@@ -406,26 +440,27 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
     for (int f = 0; f < GF_COUNT; ++f) {
         size_t nq = 0;
         for (int qi = 0; qi < GQ; ++qi) nq += qlists[f][qi].size();
-        auto& WL = wlists[f];
-        if (nq == 0 && WL.empty()) continue;
+        if (nq == 0) continue;
         GRec* dst = reinterpret_cast<GRec*>(image.data() + pos);
         GroupedDevice& D = p->fam[p->nfam];
         size_t x = 0;
         for (int qi = 0; qi < GQ; ++qi) {
             D.qoff[qi] = (int)x;
-            for (const HostItem& h : qlists[f][qi]) dst[x++] = h.rec;
+            for (const HostEntry& h : qlists[f][qi]) {
+                for (int w = 0; w < 4; ++w) dst[4 * x + (size_t)w] = h.rec[w];
+                ++x;
+            }
         }
         D.qoff[GQ] = (int)x;
-        for (size_t i = 0; i < WL.size(); ++i) dst[x + i] = WL[i].rec;
         D.recs = reinterpret_cast<const GRec*>(dev + pos);
         D.nitems = (int)nq;
-        D.nwitems = (int)WL.size();
         D.A = A;
         D.B = B;
         D.C = C;
+        D.ctrs = reinterpret_cast<int*>(dev + off_ctrs + (size_t)p->nfam * CTR_BYTES);
         p->fam_id[p->nfam] = f;
         ++p->nfam;
-        pos += (nq + WL.size()) * sizeof(GRec);
+        pos += nq * 4 * sizeof(GRec);
     }
     e = hipMemcpy(p->mem, image.data(), bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {

@@ -513,19 +513,24 @@ struct GShape {
     static constexpr int LDS = TileShape<AR, true, true, BM, BN, BK, 2, 2>::LDS_BYTES;
 };
 
-// LDS of the 128x128 tile: direct-to-LDS stages for int8 and fp32, the
-// register-staged double buffer otherwise.
+// The int8 and fp32 forms run their 64x64 and 128x128 tiles with
+// direct-to-LDS stages, the other forms register-staged double buffers.
 template <class AR>
-constexpr int g128_lds()
+constexpr bool g_dma()
 {
-    if constexpr (AR::ID == ARITH_I8 || AR::ID == ARITH_F32) return gdma_lds_bytes<AR>();
-    else return GShape<AR, GCFG_128>::LDS;
+    return AR::ID == ARITH_I8 || AR::ID == ARITH_F32;
+}
+template <class AR, int CFG>
+constexpr int g_tile_lds()
+{
+    if constexpr (g_dma<AR>() && CFG != GCFG_32) return gdma_lds_bytes<AR, CFG == GCFG_64 ? 64 : 128>();
+    else return GShape<AR, CFG>::LDS;
 }
 
 template <class AR>
 constexpr int g_lds_max()
 {
-    constexpr int a = GShape<AR, GCFG_32>::LDS, b = GShape<AR, GCFG_64>::LDS, c = g128_lds<AR>();
+    constexpr int a = g_tile_lds<AR, GCFG_32>(), b = g_tile_lds<AR, GCFG_64>(), c = g_tile_lds<AR, GCFG_128>();
     constexpr int ab = a > b ? a : b;
     return grouped_max_cfg(AR::ID) == GCFG_128 ? (ab > c ? ab : c) : ab;
 }
@@ -541,8 +546,8 @@ __device__ __forceinline__ void g_run(int cfg, const TileJob& j, uint32_t abytes
     }
     if constexpr (grouped_max_cfg(AR::ID) == GCFG_128) {
         if (cfg == GCFG_128) {
-            if constexpr (AR::ID == ARITH_I8 || AR::ID == ARITH_F32) {
-                g_run_dma<AR>(j, abytes, bbytes, smem, first_block);
+            if constexpr (g_dma<AR>()) {
+                g_run_dma<AR, 128>(j, abytes, bbytes, smem, first_block);
             } else {
                 using G = GShape<AR, GCFG_128>;
                 gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, first_block);
@@ -550,8 +555,12 @@ __device__ __forceinline__ void g_run(int cfg, const TileJob& j, uint32_t abytes
             return;
         }
     }
-    using G = GShape<AR, GCFG_64>;
-    gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, first_block);
+    if constexpr (g_dma<AR>()) {
+        g_run_dma<AR, 64>(j, abytes, bbytes, smem, first_block);
+    } else {
+        using G = GShape<AR, GCFG_64>;
+        gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, first_block);
+    }
 }
 
 // k-steps per stage of a wave item
@@ -646,34 +655,50 @@ __device__ __forceinline__ GWork g_decode(uint32_t v)
     return g;
 }
 
-// A workgroup's walk over the workgroup items: queue q = b % GQ, items
-// rank, rank + nq, rank + 2 nq, ... of it (nq workgroups take queue q, rank
-// = b / GQ); with fewer workgroups than queues, workgroup b takes queues b,
-// b + G, b + 2 G, ... (G = gridDim.x) on its own.  it < 0: no item left.
-struct GCursor {
-    int q, it, end, nq;
+// Entries are claimed from the per-XCD queues by the workgroup's leader
+// thread, one atomic per entry on the queue's counter.  Workgroup b's home
+// queue is b % GQ (b runs on XCD b % 8, whose L2 then holds the queue's
+// instances); the n workgroups of a queue take its first 3 n entries at
+// fixed places (workgroup r: entries r, r + n, r + 2 n, so a launch does not
+// start with a burst of atomics on one counter) and claim the rest one at a
+// time, moving on to the next queues when theirs runs dry, so the queues
+// drain together whatever the entries actually cost (a fixed assignment had
+// left some workgroups running 30 % longer than the median on the same
+// estimated work).  An atomic's return retires in order with the wave's
+// other memory accesses, so a claim is issued right after an entry's stores
+// and examined one entry later (g_claim_finish).
+struct GClaim {
+    int q;      // queue drawn from
+    int tried;  // queues found empty (queues only empty further)
 };
-__device__ __forceinline__ void g_queue(const GroupedDevice& d, int q, GCursor& c)
+// The number of workgroups whose home queue is q, of G dealt b % GQ.
+__device__ __forceinline__ int g_home_count(int G, int q)
 {
-    const int G = (int)gridDim.x;
-    c.q = q;
-    c.nq = G < GQ ? 1 : (G - q + GQ - 1) / GQ;
-    c.it = d.qoff[q] + (G < GQ ? 0 : (int)blockIdx.x / GQ);
-    c.end = d.qoff[q + 1];
+    return G > q ? (G - q + GQ - 1) / GQ : 0;
 }
-__device__ __forceinline__ void g_skip(const GroupedDevice& d, GCursor& c)
+__device__ __forceinline__ int g_claim_issue(int* ctr, const GClaim& c)
 {
-    const int qstep = (int)gridDim.x < GQ ? (int)gridDim.x : GQ;
-    while (c.it >= 0 && c.it >= c.end) {
-        if (c.q + qstep >= GQ) c.it = -1;
-        else g_queue(d, c.q + qstep, c);
+    return c.tried >= GQ ? INT_MAX : atomicAdd(ctr + c.q * GCTR_STRIDE, 1);
+}
+// The entry the claim `r` on queue c.q gives (past the queue's fixed
+// entries), moving on to the next queues while they are empty; -1 when
+// every queue is.
+__device__ __forceinline__ int g_claim_finish(const int* off, int* ctr, int G, GClaim& c, int r)
+{
+    while (c.tried < GQ) {
+        const int x = 3 * g_home_count(G, c.q) + r;
+        if (x < off[c.q + 1] - off[c.q]) return off[c.q] + x;
+        c.q = c.q + 1 == GQ ? 0 : c.q + 1;
+        if (++c.tried >= GQ) break;
+        r = atomicAdd(ctr + c.q * GCTR_STRIDE, 1);
     }
+    return -1;
 }
-__device__ __forceinline__ void g_next(const GroupedDevice& d, GCursor& c)
+// Fixed entry s (0, 1, 2) of workgroup `rank` of queue q (n workgroups), -1 past the queue.
+__device__ __forceinline__ int g_fixed(const int* off, int q, int n, int rank, int s)
 {
-    if (c.it < 0) return;
-    c.it += c.nq;
-    g_skip(d, c);
+    const int x = rank + s * n;
+    return x < off[q + 1] - off[q] ? off[q] + x : -1;
 }
 
 // Waves per SIMD the family's kernel is compiled for: three workgroups per
@@ -685,81 +710,104 @@ constexpr int g_waves_per_eu(int second_form)
 }
 
 // FA: the family's first form; FB: its second (== FA for one-form families).
-// Workgroup items first (the longer ones): workgroup b, on XCD b % 8, takes
-// its fixed share of queue b % GQ (the queue is sorted longest first, so
-// every workgroup gets one item of each stretch of similar cost).  Then
-// wave items, the same way over all the grid's waves.  The assignment is
-// fixed: no claims, so nothing waits on an atomic (whose return, on gfx9,
-// retires in order behind the previous tile's stores) or on the barriers
-// that shared a claimed number.
-// While an item runs, the record of the item after next and the pointers of
-// the next item are read; they land with the item's first k-block, so an
-// item starts with its description at hand instead of a chain of dependent
-// loads, each in order behind the previous item's stores.
+// Each workgroup runs the entries it claims: a tile with all four waves, or
+// a bundle of four one-wave items, one per wave (tiles and bundles of a
+// queue sorted together by cost, so small items run beside the tiles).
+// Claims run three entries ahead, and while an entry runs each wave reads
+// the record of its part of the entry after next and the operand pointers
+// of the next one, so an entry starts with its description at hand instead
+// of a chain of dependent loads, each in order behind the previous entry's
+// stores.  The last workgroup to finish re-arms the counters for the next
+// launch (a captured graph replays the kernel as it is); launches of one
+// plan must not overlap.
 template <class FA, class FB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(g_waves_per_eu(FB::ID)))) grouped_gemm(const GroupedDevice d)
 {
     constexpr int LA = g_lds_max<FA>(), LB = g_lds_max<FB>();
     __shared__ __attribute__((aligned(16))) char smem[LA > LB ? LA : LB];
-    static_assert((LA > LB ? LA : LB) >= 4 * 4096, "a 4 KiB LDS slot per wave for the wave items");
+    __shared__ int s_claim[3];
+    static_assert((LA > LB ? LA : LB) >= 4 * 4096, "a 4 KiB LDS slot per wave for the one-wave items");
     const int lane = threadIdx.x & 63;
-    if (d.nitems > 0) {
-        GCursor c0;
-        g_queue(d, (int)blockIdx.x % GQ, c0);
-        g_skip(d, c0);
-        if (c0.it >= 0) {
-            GCursor c1 = c0;
-            g_next(d, c1);
-            GCursor c2 = c1;
-            g_next(d, c2);
-            auto valid = [&](const GCursor& c) { return c.it >= 0 ? c.it : c0.it; };
-            // the first item's record, then its pointers with the next record
-            const uint32_t r0 = g_fetch(d, c0.it, 0, c0.it, lane);
-            uint32_t v = g_settle(g_fetch(d, c0.it, __builtin_amdgcn_readlane((int)r0, 0), valid(c1), lane));
-            while (c0.it >= 0) {
-                const GWork w = g_decode(v);
-                const uint32_t t = g_fetch(d, valid(c1), __builtin_amdgcn_readlane((int)v, 32), valid(c2), lane);
-                uint32_t v_next = 0u;
-                auto prefetched = [&]() { v_next = g_settle(t); };
-                if (w.form == FA::ID) g_run<FA>(w.cfg, w.j, w.abytes, w.bbytes, smem, prefetched);
-                else g_run<FB>(w.cfg, w.j, w.abytes, w.bbytes, smem, prefetched);
-                v = v_next;
-                c0 = c1;
-                c1 = c2;
-                g_next(d, c2);
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const bool leader = threadIdx.x == 0;
+    int* const qctr = d.ctrs;
+    const int G = (int)gridDim.x, home = (int)blockIdx.x % GQ, rank = (int)blockIdx.x / GQ;
+    const int nh = g_home_count(G, home);
+    GClaim cs{home, 0};
+    int i0 = g_fixed(d.qoff, home, nh, rank, 0), i1 = g_fixed(d.qoff, home, nh, rank, 1),
+        i2 = g_fixed(d.qoff, home, nh, rank, 2);
+    int cl = INT_MAX;  // the leader's claim in flight
+    if (i2 < 0) {
+        // no claims left in the home queue for this workgroup: claim at once
+        // (the fixed entries that exist come first, in order)
+        if (leader) {
+            int got[3] = {i0, i1, i2};
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+                if (got[x] < 0) got[x] = g_claim_finish(d.qoff, qctr, G, cs, g_claim_issue(qctr, cs));
+#pragma unroll
+            for (int x = 0; x < 3; ++x) s_claim[x] = got[x];
+        }
+        __syncthreads();
+        i0 = s_claim[0];
+        i1 = s_claim[1];
+        i2 = s_claim[2];
+        __syncthreads();
+    }
+    if (leader) cl = g_claim_issue(qctr, cs);
+    if (i0 >= 0) {
+        // this wave's record of entry e (a missing entry: the current one's)
+        auto rec = [&](int e) { return 4 * (e >= 0 ? e : i0) + wave; };
+        // the first entry's record, then its pointers with the next record
+        const uint32_t r0 = g_fetch(d, rec(i0), 0, rec(i0), lane);
+        uint32_t v = g_settle(g_fetch(d, rec(i0), __builtin_amdgcn_readlane((int)r0, 0), rec(i1), lane));
+        int slot = 0;
+        while (i0 >= 0) {
+            const GWork w = g_decode(v);
+            const uint32_t t = g_fetch(d, rec(i1), __builtin_amdgcn_readlane((int)v, 32), rec(i2), lane);
+            uint32_t v_next = 0u;
+            auto loaded = [&]() {
+                v_next = g_settle(t);
+                if (leader) s_claim[slot] = g_claim_finish(d.qoff, qctr, G, cs, cl);
+            };
+            if (w.cfg == GCFG_W32) {
+                if (w.j.m > 0) {
+                    char* slotp = smem + 4096 * wave;
+                    if (w.form == FA::ID) g_run_wave<FA>(w.j, w.abytes, w.bbytes, slotp);
+                    else g_run_wave<FB>(w.j, w.abytes, w.bbytes, slotp);
+                }
+                loaded();
+            } else {
+                if (w.form == FA::ID) g_run<FA>(w.cfg, w.j, w.abytes, w.bbytes, smem, loaded);
+                else g_run<FB>(w.cfg, w.j, w.abytes, w.bbytes, smem, loaded);
             }
+            if (leader) cl = g_claim_issue(qctr, cs);  // for the entry four ahead
+            __syncthreads();  // publishes the claim; the entry's LDS readers are done
+            const int i3 = s_claim[slot];
+            slot ^= 1;
+            v = v_next;
+            i0 = i1;
+            i1 = i2;
+            i2 = i3;
         }
     }
-    if (d.nwitems > 0) {
-        // the workgroup's last tile may still be read from LDS by another wave
-        lds_barrier();
-        const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-        const int nw = 4 * (int)gridDim.x;
-        const int w0 = 4 * (int)blockIdx.x + wave;
-        if (w0 < d.nwitems) {
-            auto idx = [&](long w) { return d.nitems + (w < d.nwitems ? (int)w : w0); };
-            const uint32_t r0 = g_fetch(d, idx(w0), 0, idx(w0), lane);
-            uint32_t v = g_settle(g_fetch(d, idx(w0), __builtin_amdgcn_readlane((int)r0, 0), idx((long)w0 + nw), lane));
-            for (int w = w0; w < d.nwitems; w += nw) {
-                const GWork g = g_decode(v);
-                const uint32_t t = g_fetch(d, idx((long)w + nw), __builtin_amdgcn_readlane((int)v, 32),
-                                           idx((long)w + 2 * (long)nw), lane);
-                char* slot = smem + 4096 * wave;
-                if (g.form == FA::ID) g_run_wave<FA>(g.j, g.abytes, g.bbytes, slot);
-                else g_run_wave<FB>(g.j, g.abytes, g.bbytes, slot);
-                v = g_settle(t);
-            }
-        }
+    // every claim of the workgroup has returned (the last one is never
+    // examined); then the last workgroup re-arms the counters
+    wait_vm<0>();
+    __syncthreads();
+    if (leader && atomicAdd(d.ctrs + GCTR_DONE * GCTR_STRIDE, 1) == G - 1) {
+        for (int x = 0; x < GCTR_COUNT; ++x)
+            __hip_atomic_store(d.ctrs + x * GCTR_STRIDE, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 template <class FA, class FB>
 hipError_t run_grouped(const GroupedDevice& d, hipStream_t st)
 {
-    if (d.nitems + d.nwitems <= 0) return hipSuccess;
+    if (d.nitems <= 0) return hipSuccess;
     auto kern = grouped_gemm<FA, FB>;
     const long slots = (long)kernel_occupancy(kern, 256) * device_cu_count();
-    const long want = (long)d.nitems + ((long)d.nwitems + 3) / 4;  // four waves take wave items at once
+    const long want = (long)d.nitems;
     const long grid = want < slots ? want : slots;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, d);
     return hipGetLastError();
