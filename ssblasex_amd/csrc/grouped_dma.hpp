@@ -348,13 +348,14 @@ struct GDmaTile {
     }
 };
 
-// One TxT tile (T = 64 or 128) of a grouped int8 / fp32 instance; `done_loading` runs
-// once the tile's loads have all landed (the item loop settles the next
+// One TxT tile (T = 64 or 128) of a grouped int8 / fp32 instance; `started`
+// runs once the first k-blocks' loads are issued, `done_loading` once the
+// tile's loads have all landed (the item loop settles the next
 // item's prefetched description there).  Starts with a barrier: the
 // previous item's LDS readers are done.
-template <class AR, int TS, bool AKC, bool BKC, class Hook>
+template <class AR, int TS, bool AKC, bool BKC, class Started, class Hook>
 __device__ __forceinline__ void gdma_job(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
-                                         Hook&& done_loading)
+                                         Started&& started, Hook&& done_loading)
 {
     constexpr int BK = GDmaCfg<AR, TS>::BK, S = GDmaCfg<AR, TS>::S;
     using T = GDmaTile<AR, AKC, BKC, TS, TS, BK, S>;
@@ -384,6 +385,7 @@ __device__ __forceinline__ void gdma_job(const TileJob& j, uint32_t abytes, uint
 #pragma unroll
     for (int s = 0; s < S - 1; ++s)
         if (s < nk) T::issue(j, ra, rb, s, smem + s * T::STAGE, wave, lane);
+    started();
     for (int kt = 0; kt < nk; ++kt) {
         char* cur = smem + (kt % S) * T::STAGE;
         T::wait_block(min(nk - 1 - kt, S - 2));
@@ -439,16 +441,16 @@ constexpr int gdma_lds_bytes()
     return T::LDS_BYTES > U::LDS_BYTES ? T::LDS_BYTES : U::LDS_BYTES;
 }
 
-template <class AR, int TS, class Hook>
+template <class AR, int TS, class Started, class Hook>
 __device__ __forceinline__ void g_run_dma(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
-                                          Hook&& done_loading)
+                                          Started&& started, Hook&& done_loading)
 {
     if (j.akc) {
-        if (j.bkc) gdma_job<AR, TS, true, true>(j, abytes, bbytes, smem, done_loading);
-        else gdma_job<AR, TS, true, false>(j, abytes, bbytes, smem, done_loading);
+        if (j.bkc) gdma_job<AR, TS, true, true>(j, abytes, bbytes, smem, started, done_loading);
+        else gdma_job<AR, TS, true, false>(j, abytes, bbytes, smem, started, done_loading);
     } else {
-        if (j.bkc) gdma_job<AR, TS, false, true>(j, abytes, bbytes, smem, done_loading);
-        else gdma_job<AR, TS, false, false>(j, abytes, bbytes, smem, done_loading);
+        if (j.bkc) gdma_job<AR, TS, false, true>(j, abytes, bbytes, smem, started, done_loading);
+        else gdma_job<AR, TS, false, false>(j, abytes, bbytes, smem, started, done_loading);
     }
 }
 

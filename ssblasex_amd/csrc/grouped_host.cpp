@@ -409,7 +409,30 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
                 }
                 E.push_back(e);
             }
-            std::stable_sort(E.begin(), E.end(), [](const HostEntry& a, const HostEntry& b) { return a.ns > b.ns; });
+            // Each form's entries longest first, the forms merged so that each
+            // has done the same fraction of its estimated work at every point:
+            // the int8 entries (bound by memory) then run beside the fp32 ones
+            // (bound by the MFMA pipes) instead of after them.
+            auto by_ns = [](const HostEntry& a, const HostEntry& b) { return a.ns > b.ns; };
+            std::vector<HostEntry> part[2];
+            double tot[2] = {0.0, 0.0};
+            for (const HostEntry& h : E) {
+                const int x = (int)((h.rec[0].bits >> 2) & 7u) == K_I8 ? 1 : 0;
+                part[x].push_back(h);
+                tot[x] += h.ns;
+            }
+            for (auto& P : part) std::stable_sort(P.begin(), P.end(), by_ns);
+            E.clear();
+            size_t at[2] = {0, 0};
+            double done[2] = {0.0, 0.0};
+            while (at[0] < part[0].size() || at[1] < part[1].size()) {
+                int x;
+                if (at[0] >= part[0].size()) x = 1;
+                else if (at[1] >= part[1].size()) x = 0;
+                else x = (done[0] + part[0][at[0]].ns) / tot[0] <= (done[1] + part[1][at[1]].ns) / tot[1] ? 0 : 1;
+                done[x] += part[x][at[x]].ns;
+                E.push_back(part[x][at[x]++]);
+            }
             nitems += (long)E.size();
         }
     }

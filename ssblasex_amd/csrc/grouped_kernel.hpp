@@ -408,9 +408,9 @@ __device__ __forceinline__ void g_wave_job(const TileJob& j, uint32_t abytes, ui
 // One BM x BN tile of one grouped instance, 4 waves (2 x 2), LDS
 // double-buffered, global loads of k-block kt+1 in flight while kt is
 // multiplied.  abytes / bbytes: extent of the stored A / B in bytes.
-template <class AR, int BM, int BN, int BK, class Hook>
+template <class AR, int BM, int BN, int BK, class Started, class Hook>
 __device__ __forceinline__ void gtile_job(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
-                                          Hook&& first_block)
+                                          Started&& started, Hook&& first_block)
 {
     using S = TileShape<AR, true, true, BM, BN, BK, 2, 2>;
     using Out = typename AR::Out;
@@ -472,6 +472,7 @@ __device__ __forceinline__ void gtile_job(const TileJob& j, uint32_t abytes, uin
     };
 
     load_step(0);
+    started();
     __syncthreads();  // previous user of smem is done
     store_step(0, 0);
     __syncthreads();
@@ -535,31 +536,31 @@ constexpr int g_lds_max()
     return grouped_max_cfg(AR::ID) == GCFG_128 ? (ab > c ? ab : c) : ab;
 }
 
-template <class AR, class Hook>
+template <class AR, class Started, class Hook>
 __device__ __forceinline__ void g_run(int cfg, const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
-                                      Hook&& first_block)
+                                      Started&& started, Hook&& first_block)
 {
     if (cfg == GCFG_32) {
         using G = GShape<AR, GCFG_32>;
-        gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, first_block);
+        gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, started, first_block);
         return;
     }
     if constexpr (grouped_max_cfg(AR::ID) == GCFG_128) {
         if (cfg == GCFG_128) {
             if constexpr (g_dma<AR>()) {
-                g_run_dma<AR, 128>(j, abytes, bbytes, smem, first_block);
+                g_run_dma<AR, 128>(j, abytes, bbytes, smem, started, first_block);
             } else {
                 using G = GShape<AR, GCFG_128>;
-                gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, first_block);
+                gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, started, first_block);
             }
             return;
         }
     }
     if constexpr (g_dma<AR>()) {
-        g_run_dma<AR, 64>(j, abytes, bbytes, smem, first_block);
+        g_run_dma<AR, 64>(j, abytes, bbytes, smem, started, first_block);
     } else {
         using G = GShape<AR, GCFG_64>;
-        gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, first_block);
+        gtile_job<AR, G::BM, G::BN, G::BK>(j, abytes, bbytes, smem, started, first_block);
     }
 }
 
@@ -665,8 +666,8 @@ __device__ __forceinline__ GWork g_decode(uint32_t v)
 // drain together whatever the entries actually cost (a fixed assignment had
 // left some workgroups running 30 % longer than the median on the same
 // estimated work).  An atomic's return retires in order with the wave's
-// other memory accesses, so a claim is issued right after an entry's stores
-// and examined one entry later (g_claim_finish).
+// other memory accesses, so a claim is issued right after an entry's first
+// loads and examined once all of them have landed (g_claim_finish).
 struct GClaim {
     int q;      // queue drawn from
     int tried;  // queues found empty (queues only empty further)
@@ -736,7 +737,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(g_wave
     GClaim cs{home, 0};
     int i0 = g_fixed(d.qoff, home, nh, rank, 0), i1 = g_fixed(d.qoff, home, nh, rank, 1),
         i2 = g_fixed(d.qoff, home, nh, rank, 2);
-    int cl = INT_MAX;  // the leader's claim in flight
     if (i2 < 0) {
         // no claims left in the home queue for this workgroup: claim at once
         // (the fixed entries that exist come first, in order)
@@ -749,12 +749,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(g_wave
             for (int x = 0; x < 3; ++x) s_claim[x] = got[x];
         }
         __syncthreads();
-        i0 = s_claim[0];
-        i1 = s_claim[1];
-        i2 = s_claim[2];
+        i0 = __builtin_amdgcn_readfirstlane(s_claim[0]);
+        i1 = __builtin_amdgcn_readfirstlane(s_claim[1]);
+        i2 = __builtin_amdgcn_readfirstlane(s_claim[2]);
         __syncthreads();
     }
-    if (leader) cl = g_claim_issue(qctr, cs);
     if (i0 >= 0) {
         // this wave's record of entry e (a missing entry: the current one's)
         auto rec = [&](int e) { return 4 * (e >= 0 ? e : i0) + wave; };
@@ -765,12 +764,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(g_wave
         while (i0 >= 0) {
             const GWork w = g_decode(v);
             const uint32_t t = g_fetch(d, rec(i1), __builtin_amdgcn_readlane((int)v, 32), rec(i2), lane);
+            // the claim of the entry three ahead: issued once the entry's first
+            // loads are, examined once they have all landed (a claim left in
+            // flight across the loop's back edge had the compiler wait for
+            // every access in flight, stores included, at the loop's head)
             uint32_t v_next = 0u;
+            int cl = INT_MAX;
+            auto started = [&]() {
+                if (leader) cl = g_claim_issue(qctr, cs);
+            };
             auto loaded = [&]() {
                 v_next = g_settle(t);
                 if (leader) s_claim[slot] = g_claim_finish(d.qoff, qctr, G, cs, cl);
             };
             if (w.cfg == GCFG_W32) {
+                started();
                 if (w.j.m > 0) {
                     char* slotp = smem + 4096 * wave;
                     if (w.form == FA::ID) g_run_wave<FA>(w.j, w.abytes, w.bbytes, slotp);
@@ -778,12 +786,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(g_wave
                 }
                 loaded();
             } else {
-                if (w.form == FA::ID) g_run<FA>(w.cfg, w.j, w.abytes, w.bbytes, smem, loaded);
-                else g_run<FB>(w.cfg, w.j, w.abytes, w.bbytes, smem, loaded);
+                if (w.form == FA::ID) g_run<FA>(w.cfg, w.j, w.abytes, w.bbytes, smem, started, loaded);
+                else g_run<FB>(w.cfg, w.j, w.abytes, w.bbytes, smem, started, loaded);
             }
-            if (leader) cl = g_claim_issue(qctr, cs);  // for the entry four ahead
-            __syncthreads();  // publishes the claim; the entry's LDS readers are done
-            const int i3 = s_claim[slot];
+            // publishes the claim (the leader's LDS write has landed) and tells
+            // every wave that the entry's LDS readers are done; not
+            // __syncthreads, whose fence would also wait for the entry's stores
+            wait_lgkm0();
+            lds_barrier();
+            // entry indices kept in scalar registers: a vector copy at the
+            // loop's end had the compiler wait for every memory access in
+            // flight (the entry's stores) before it
+            const int i3 = __builtin_amdgcn_readfirstlane(s_claim[slot]);
             slot ^= 1;
             v = v_next;
             i0 = i1;
@@ -791,8 +805,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(g_wave
             i2 = i3;
         }
     }
-    // every claim of the workgroup has returned (the last one is never
-    // examined); then the last workgroup re-arms the counters
+    // every claim of the workgroup has returned; then the last workgroup
+    // re-arms the counters
     wait_vm<0>();
     __syncthreads();
     if (leader && atomicAdd(d.ctrs + GCTR_DONE * GCTR_STRIDE, 1) == G - 1) {
