@@ -378,51 +378,50 @@ ssblasStatus_t grouped_create(ssblasGroupedPlan_t* plan, long group_count, const
         double ns;
     };
     std::vector<HostEntry> qlists[GF_COUNT][GQ];
-    auto by_cost = [](const HostItem& a, const HostItem& b) { return a.ns > b.ns; };
     long nitems = 0;
     for (int f = 0; f < GF_COUNT; ++f) {
         for (int qi = 0; qi < GQ; ++qi) {
-            std::vector<HostItem>& Q = qitems[f][qi];
-            std::stable_sort(Q.begin(), Q.end(), by_cost);
-            std::vector<HostEntry>& E = qlists[f][qi];
-            std::vector<const HostItem*> waves;
-            for (const HostItem& h : Q) {
+            // The queue's items are instance by instance, longest instance
+            // first, each instance's tiles before its strips: kept in that
+            // order, so that the entries of one instance run close together
+            // in time and its operands are read from HBM into the XCD's L2
+            // once.  One sequence per form (int8, the others), bundles of
+            // four consecutive one-wave items of a form.
+            std::vector<HostEntry> part[2];
+            double tot[2] = {0.0, 0.0};
+            HostEntry bundle[2];
+            int nb[2] = {0, 0};
+            auto close_bundle = [&](int x) {
+                for (int w = nb[x]; w < 4; ++w) {
+                    bundle[x].rec[w] = bundle[x].rec[0];
+                    bundle[x].rec[w].m = bundle[x].rec[w].n = bundle[x].rec[w].k = 0;
+                }
+                part[x].push_back(bundle[x]);
+                tot[x] += bundle[x].ns;
+                nb[x] = 0;
+            };
+            for (const HostItem& h : qitems[f][qi]) {
+                const int x = (int)((h.rec.bits >> 2) & 7u) == K_I8 ? 1 : 0;
                 if ((h.rec.bits & 3u) == (uint32_t)GCFG_W32) {
-                    waves.push_back(&h);
+                    if (nb[x] == 0) bundle[x].ns = 0.0;
+                    bundle[x].rec[nb[x]++] = h.rec;
+                    bundle[x].ns = std::max(bundle[x].ns, h.ns);
+                    if (nb[x] == 4) close_bundle(x);
                 } else {
                     HostEntry e;
                     for (int w = 0; w < 4; ++w) e.rec[w] = h.rec;
                     e.ns = h.ns;
-                    E.push_back(e);
+                    part[x].push_back(e);
+                    tot[x] += e.ns;
                 }
             }
-            for (size_t x = 0; x < waves.size(); x += 4) {
-                HostEntry e;
-                e.ns = waves[x]->ns;  // the longest of the four
-                for (int w = 0; w < 4; ++w) {
-                    if (x + (size_t)w < waves.size()) {
-                        e.rec[w] = waves[x + (size_t)w]->rec;
-                    } else {
-                        e.rec[w] = waves[x]->rec;
-                        e.rec[w].m = e.rec[w].n = e.rec[w].k = 0;
-                    }
-                }
-                E.push_back(e);
-            }
-            // Each form's entries longest first, the forms merged so that each
-            // has done the same fraction of its estimated work at every point:
-            // the int8 entries (bound by memory) then run beside the fp32 ones
-            // (bound by the MFMA pipes) instead of after them.
-            auto by_ns = [](const HostEntry& a, const HostEntry& b) { return a.ns > b.ns; };
-            std::vector<HostEntry> part[2];
-            double tot[2] = {0.0, 0.0};
-            for (const HostEntry& h : E) {
-                const int x = (int)((h.rec[0].bits >> 2) & 7u) == K_I8 ? 1 : 0;
-                part[x].push_back(h);
-                tot[x] += h.ns;
-            }
-            for (auto& P : part) std::stable_sort(P.begin(), P.end(), by_ns);
-            E.clear();
+            for (int x = 0; x < 2; ++x)
+                if (nb[x] > 0) close_bundle(x);
+            // the forms merged so that each has done the same fraction of its
+            // estimated work at every point: the int8 entries (bound by
+            // memory) run beside the fp32 ones (bound by the MFMA pipes)
+            // instead of after them
+            std::vector<HostEntry>& E = qlists[f][qi];
             size_t at[2] = {0, 0};
             double done[2] = {0.0, 0.0};
             while (at[0] < part[0].size() || at[1] < part[1].size()) {
