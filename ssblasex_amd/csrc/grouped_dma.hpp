@@ -31,70 +31,9 @@
 // outputs that are never stored.
 #pragma once
 #include "gemm_dma_kc.hpp"
+#include "wave_job.hpp"
 
 namespace ssb {
-
-// Elements from index `nvalid` on of a word (WB bytes, E-byte elements) set to 0.
-template <int WB, int E>
-__device__ __forceinline__ void clear_tail(uint32_t* d, int nvalid)
-{
-    const int vb = nvalid * E;
-#pragma unroll
-    for (int i = 0; i < WB / 4; ++i) {
-        const int lo = 4 * i;
-        const uint32_t m = vb >= lo + 4 ? 0xffffffffu : (vb <= lo ? 0u : ((1u << (8 * (vb - lo))) - 1u));
-        d[i] &= m;
-    }
-}
-
-template <int AL>
-struct ElemAligned16;
-template <>
-struct ElemAligned16<4> {
-    typedef uint32_t T __attribute__((ext_vector_type(4), aligned(4)));
-};
-template <>
-struct ElemAligned16<8> {
-    typedef uint32_t T __attribute__((ext_vector_type(4), aligned(8)));
-};
-
-// Fused alpha/beta epilogue of one 16x16 fragment at any ldc and alignment:
-// the lane's 4 rows go out as one (unaligned) vector when all are in range.
-template <class AR>
-__device__ __forceinline__ void gstore_frag(typename AR::Out* C, long ldc, int r0, int c,
-                                            const typename AR::Acc& acc, const Scalars& sc, bool beta0, int m,
-                                            int n)
-{
-    using Out = typename AR::Out;
-    // 16 bytes at element (not vector) alignment
-    using u32x4_e = typename ElemAligned16<sizeof(Out)>::T;
-    constexpr int NV = 4 * (int)sizeof(Out) / 16;
-    if (c >= n) return;
-    Out* dst = C + (long)c * ldc + r0;
-    if (r0 + 4 <= m) {
-        u32x4_e w[NV];
-        Out old[4] = {Out(0), Out(0), Out(0), Out(0)};
-        if (!beta0) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) w[i] = *gbl(reinterpret_cast<const u32x4_e*>(dst) + i);
-            __builtin_memcpy(old, w, sizeof(old));
-        }
-        Out res[4];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) res[v] = epi(acc[v], old[v], sc, beta0);
-        __builtin_memcpy(w, res, sizeof(res));
-#pragma unroll
-        for (int i = 0; i < NV; ++i) *gbl(reinterpret_cast<u32x4_e*>(dst) + i) = w[i];
-    } else {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            if (r0 + v < m) {
-                const Out o = beta0 ? Out(0) : *gbl(dst + v);
-                *gbl(dst + v) = epi(acc[v], o, sc, beta0);
-            }
-        }
-    }
-}
 
 // fp32 M/N-contiguous operand in LDS: row = k, 16-byte chunks of 4
 // consecutive M/N values; chunk (row, c) at slot row*CP + (c ^ swz(row)).

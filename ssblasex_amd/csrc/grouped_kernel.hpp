@@ -205,206 +205,6 @@ struct GStage {
     }
 };
 
-typedef int32_t gi32x2 __attribute__((ext_vector_type(2)));
-typedef __attribute__((address_space(3))) gi32x2 g_lds_i32x2;
-
-// One operand of a wave item (32 rows of op(X): two 16-row MFMA fragments)
-// read straight into fragment registers through a buffer descriptor over the
-// stored matrix.  Lane l supplies row l%16 of a fragment and, within a k-step
-// of KSTEP elements, the KL = KSTEP/4 elements from k = (l/16) KL on:
-//  * K-contiguous: one 16-byte word per fragment; in the step that reaches
-//    past K its elements at k >= K (the next row) are cleared;
-//  * fp32 M/N-contiguous: four dwords per fragment, one per k; a dword at
-//    k >= K starts past the end of the matrix and reads as zero;
-//  * int8 M/N-contiguous: lane l reads the 16 bytes of rows 0..15 at k = l,
-//    writes them to the wave's LDS slot and reads its fragment back with two
-//    ds_read_b64_tr_b8 (output lane o of a 16-lane group receives column o
-//    of the 8 consecutive rows its group's lanes point at).
-// int8 words that cross the end of the matrix read as zero from the dword on
-// that holds the last bytes; they are read again as the 16 bytes ending at
-// the end and shifted (only where a lane of the wave holds one).
-template <class AR, bool KC>
-struct GWaveOp {
-    static constexpr int E = AR::E, KL = AR::KSTEP / 4;
-    static_assert(KL * E == 16, "one 16-byte word per lane and fragment");
-
-    // Byte offset of the word lane `lane` reads for fragment f at step kb.
-    __device__ __forceinline__ static uint32_t off(int r0, int f, int kb, int ld, int lane, int s = 0)
-    {
-        if constexpr (KC) return (uint32_t)(((r0 + 16 * f + (lane & 15)) * ld + kb + (lane >> 4) * KL) * E);
-        else if constexpr (E == 4) return (uint32_t)(((kb + (lane >> 4) * KL + s) * ld + r0 + 16 * f + (lane & 15)) * E);
-        else return (uint32_t)((kb + lane) * ld + r0 + 16 * f);
-    }
-
-    __device__ __forceinline__ static void load(u32x4* w, __amdgpu_buffer_rsrc_t rs, int r0, int kb, int ld,
-                                                int lane)
-    {
-#pragma unroll
-        for (int f = 0; f < 2; ++f) {
-            if constexpr (!KC && E == 4) {
-#pragma unroll
-                for (int s = 0; s < 4; ++s) w[f][s] = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)off(r0, f, kb, ld, lane, s), 0, 0);
-            } else {
-                w[f] = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)off(r0, f, kb, ld, lane), 0, 0);
-            }
-        }
-    }
-
-    // int8 words crossing the end of the matrix (`ext` bytes), and the k tail
-    // of a K-contiguous operand (`kv` valid elements from kb on).
-    __device__ __forceinline__ static void fix(u32x4* w, __amdgpu_buffer_rsrc_t rs, int r0, int kb, int ld, int lane,
-                                               uint32_t ext, int kv)
-    {
-        if constexpr (E == 1) {
-            bool any = false;
-#pragma unroll
-            for (int f = 0; f < 2; ++f) {
-                const uint32_t o = off(r0, f, kb, ld, lane);
-                any |= o < ext && o + 16u > ext;
-            }
-            if (__any(any)) {
-#pragma unroll
-                for (int f = 0; f < 2; ++f) {
-                    const uint32_t o = off(r0, f, kb, ld, lane);
-                    uint32_t d[4] = {w[f][0], w[f][1], w[f][2], w[f][3]};
-                    if (o < ext && o + 16u > ext) {
-                        if (ext < 16u) {
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) d[i] = 0u;
-                            for (uint32_t p = o; p < ext; ++p)
-                                d[(p - o) >> 2] |= __builtin_amdgcn_raw_buffer_load_b8(rs, (int)p, 0, 0)
-                                                   << (8 * ((p - o) & 3));
-                        } else {
-                            const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)(ext - 16u), 0, 0);
-                            const uint32_t sh = o - (ext - 16u), q = sh >> 2, r = sh & 3;
-                            const uint32_t src[4] = {v[0], v[1], v[2], v[3]};
-                            uint32_t t[5];
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) {
-                                const uint32_t a = i + q;
-                                t[i] = a == 0 ? src[0] : a == 1 ? src[1] : a == 2 ? src[2] : a == 3 ? src[3] : 0u;
-                            }
-                            t[4] = 0u;
-#pragma unroll
-                            for (int i = 0; i < 4; ++i) d[i] = __builtin_amdgcn_alignbyte(t[i + 1], t[i], r);
-                        }
-                    }
-                    w[f] = u32x4{d[0], d[1], d[2], d[3]};
-                }
-            }
-        }
-        if constexpr (KC) {
-            if (kv < AR::KSTEP) {
-#pragma unroll
-                for (int f = 0; f < 2; ++f) {
-                    uint32_t d[4] = {w[f][0], w[f][1], w[f][2], w[f][3]};
-                    clear_tail<16, E>(d, min(max(kv - (lane >> 4) * KL, 0), KL));
-                    w[f] = u32x4{d[0], d[1], d[2], d[3]};
-                }
-            }
-        }
-    }
-
-    // Words -> MFMA fragments (int8 M/N-contiguous: through the wave's 2 KiB LDS slot).
-    __device__ __forceinline__ static void frags(u32x4* w, char* slot, int lane)
-    {
-        if constexpr (!KC && E == 1) {
-#pragma unroll
-            for (int f = 0; f < 2; ++f) *reinterpret_cast<u32x4*>(slot + 1024 * f + 16 * lane) = w[f];
-            const int r = lane & 15, g = lane >> 4;
-#pragma unroll
-            for (int f = 0; f < 2; ++f) {
-                gi32x2 v[2];
-#pragma unroll
-                for (int h = 0; h < 2; ++h)
-                    v[h] = __builtin_amdgcn_ds_read_tr8_b64_v2i32(
-                        (g_lds_i32x2*)(slot + 1024 * f + 16 * (16 * g + 8 * h + (r >> 1)) + 8 * (r & 1)));
-                w[f] = u32x4{(uint32_t)v[0].x, (uint32_t)v[0].y, (uint32_t)v[1].x, (uint32_t)v[1].y};
-            }
-        }
-    }
-};
-
-// One wave's 32x32 block of a grouped instance (W32 items) for the int8 and
-// fp32 forms: fragments loaded straight from global memory through buffer
-// descriptors, NS k-steps per stage, the next stage's loads in flight while
-// the current one is multiplied (two register sets, the loop unrolled by
-// two so no set is copied while its loads are in flight).  `slot`: 4 KiB of
-// LDS private to the wave.
-template <class AR, bool AKC, bool BKC, int NS>
-__device__ __forceinline__ void g_wave_job(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* slot)
-{
-    using Out = typename AR::Out;
-    using OA = GWaveOp<AR, AKC>;
-    using OB = GWaveOp<AR, BKC>;
-    constexpr int KS = AR::KSTEP;
-    int lane = threadIdx.x & 63;
-    __asm__ volatile("" : "+v"(lane));
-    const __amdgpu_buffer_rsrc_t ra = make_rsrc(j.A, abytes);
-    const __amdgpu_buffer_rsrc_t rb = make_rsrc(j.B, bbytes);
-    const int lda = (int)j.lda, ldb = (int)j.ldb;
-    const int nsteps = (j.k + KS - 1) / KS;
-    const int nst = (nsteps + NS - 1) / NS;
-
-    typename AR::Acc acc[2][2];
-#pragma unroll
-    for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-        for (int fn = 0; fn < 2; ++fn) acc[fm][fn] = typename AR::Acc{};
-
-    u32x4 a0[NS][2], b0[NS][2], a1[NS][2], b1[NS][2];
-    auto load = [&](u32x4 (*a)[2], u32x4 (*b)[2], int s) {
-#pragma unroll
-        for (int t = 0; t < NS; ++t) {
-            const int kb = (s * NS + t) * KS;
-            if (kb < j.k) {
-                OA::load(a[t], ra, j.i0, kb, lda, lane);
-                OB::load(b[t], rb, j.j0, kb, ldb, lane);
-            }
-        }
-    };
-    auto run = [&](u32x4 (*a)[2], u32x4 (*b)[2], int s) {
-#pragma unroll
-        for (int t = 0; t < NS; ++t) {
-            const int kb = (s * NS + t) * KS;
-            if (kb < j.k) {
-                OA::fix(a[t], ra, j.i0, kb, lda, lane, abytes, j.k - kb);
-                OB::fix(b[t], rb, j.j0, kb, ldb, lane, bbytes, j.k - kb);
-                OA::frags(a[t], slot, lane);
-                OB::frags(b[t], slot + 2048, lane);
-#pragma unroll
-                for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-                    for (int fn = 0; fn < 2; ++fn) AR::mma(acc[fm][fn], &a[t][fm], &b[t][fn]);
-            }
-        }
-    };
-    if (nst > 0) load(a0, b0, 0);
-    for (int s = 0; s < nst; s += 2) {
-        if (s + 1 < nst) load(a1, b1, s + 1);
-        run(a0, b0, s);
-        if (s + 1 >= nst) break;
-        if (s + 2 < nst) load(a0, b0, s + 2);
-        run(a1, b1, s + 1);
-    }
-
-    Out* C = static_cast<Out*>(j.C);
-    const int row0 = j.i0 + 4 * (lane >> 4), col0 = j.j0 + (lane & 15);
-    if (j.beta0 != 0) {
-#pragma unroll
-        for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < 2; ++fn)
-                gstore_frag<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, true, j.m, j.n);
-    } else {
-#pragma unroll
-        for (int fm = 0; fm < 2; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < 2; ++fn)
-                gstore_frag<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, false, j.m, j.n);
-    }
-}
-
 // One BM x BN tile of one grouped instance, 4 waves (2 x 2), LDS
 // double-buffered, global loads of k-block kt+1 in flight while kt is
 // multiplied.  abytes / bbytes: extent of the stored A / B in bytes.
@@ -575,11 +375,11 @@ __device__ __forceinline__ void g_run_wave(const TileJob& j, uint32_t abytes, ui
     if constexpr (AR::ID == ARITH_I8 || AR::ID == ARITH_F32) {
         constexpr int NS = G_WAVE_NS;
         if (j.akc) {
-            if (j.bkc) g_wave_job<AR, true, true, NS>(j, abytes, bbytes, slot);
-            else g_wave_job<AR, true, false, NS>(j, abytes, bbytes, slot);
+            if (j.bkc) wave_job<AR, true, true, 2, 2, NS, false>(j, abytes, bbytes, slot);
+            else wave_job<AR, true, false, 2, 2, NS, false>(j, abytes, bbytes, slot);
         } else {
-            if (j.bkc) g_wave_job<AR, false, true, NS>(j, abytes, bbytes, slot);
-            else g_wave_job<AR, false, false, NS>(j, abytes, bbytes, slot);
+            if (j.bkc) wave_job<AR, false, true, 2, 2, NS, false>(j, abytes, bbytes, slot);
+            else wave_job<AR, false, false, 2, 2, NS, false>(j, abytes, bbytes, slot);
         }
     } else {
         if (j.akc) {

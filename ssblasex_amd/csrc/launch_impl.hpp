@@ -7,6 +7,7 @@
 #include "gemm_dma_kc.hpp"
 #include "gemm_kernels.hpp"
 #include "launch.hpp"
+#include "wave_job.hpp"
 #include "occupancy.hpp"
 
 namespace ssb {
@@ -140,16 +141,33 @@ hipError_t run_kc(KParams p, hipStream_t st)
     }
 }
 
+// One wave per W x W block.  int8, and fp32 at W = 32: buffer loads
+// straight into the fragments, k-steps staged two register sets deep, old C
+// read before the k-loop (int8 32^3 x 65536: 150 -> 90 us; fp32 32^3 x 65536:
+// 163-188 -> 160-166 us).  fp32 at W = 16 (where the staged path held more
+// registers and was slower: 64^3 x 128 4.7 -> 5.3 us), the other forms, and
+// matrices whose extent does not fit the descriptors' 31-bit offsets:
+// guarded element loads.
 template <class AR, bool AKC, bool BKC, int W>
 hipError_t run_direct(KParams p, hipStream_t st)
 {
-    auto kern = direct_gemm<AR, AKC, BKC, W, W>;
     p.tiles_m = (p.m + W - 1) / W;
     p.tiles_n = (p.n + W - 1) / W;
     p.total = (long)p.batch * p.tiles_m * p.tiles_n;
-    int occ = kernel_occupancy(kern, 256);
-    long blocks = (p.total + 3) / 4;
-    long grid = persistent_grid(blocks, occ);
+    const long blocks = (p.total + 3) / 4;
+    if constexpr (AR::ID == ARITH_I8 || (AR::ID == ARITH_F32 && W == 32)) {
+        constexpr int E = AR::E;
+        const uint32_t abytes = AKC ? wave_extent(p.k, p.m, p.lda, E) : wave_extent(p.m, p.k, p.lda, E);
+        const uint32_t bbytes = BKC ? wave_extent(p.k, p.n, p.ldb, E) : wave_extent(p.n, p.k, p.ldb, E);
+        if (abytes != 0u && bbytes != 0u) {
+            auto kern = wave_gemm<AR, AKC, BKC, W, W, 2>;
+            const long grid = persistent_grid(blocks, kernel_occupancy(kern, 256));
+            hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, p, abytes, bbytes);
+            return hipGetLastError();
+        }
+    }
+    auto kern = direct_gemm<AR, AKC, BKC, W, W>;
+    const long grid = persistent_grid(blocks, kernel_occupancy(kern, 256));
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, p);
     return hipGetLastError();
 }
