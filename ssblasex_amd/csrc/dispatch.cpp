@@ -37,6 +37,7 @@ const char* config_name(int cfg)
     case CFG_DMA_256: return "dma256";
     case CFG_DMA_256_W16: return "dma256w16";
     case CFG_DMA_256_K2: return "dma256k2";
+    case CFG_GDMA_128: return "gdma128";
     default: return "unknown";
     }
 }
@@ -47,7 +48,8 @@ void config_tile(int cfg, int* tm, int* tn)
                                            {64, 128},  {128, 128}, {128, 128}, {256, 128}, {64, 64},
                                            {128, 128}, {128, 128}, {128, 128}, {256, 128}, {64, 64},
                                            {128, 128}, {128, 128}, {256, 128}, {128, 128},
-                                           {256, 128}, {128, 128}, {256, 256}, {256, 256}, {256, 256}};
+                                           {256, 128}, {128, 128}, {256, 256}, {256, 256}, {256, 256},
+                                           {128, 128}};
     if (cfg < 0 || cfg >= CFG_COUNT) cfg = CFG_TILE_64;
     *tm = dims[cfg][0];
     *tn = dims[cfg][1];

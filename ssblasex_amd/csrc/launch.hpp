@@ -49,7 +49,11 @@ enum TileConfig : int {
     CFG_DMA_256 = 21,       // 8 waves of 128x64
     CFG_DMA_256_W16 = 22,   // 16 waves of 64x64
     CFG_DMA_256_K2 = 23,    // 8 waves of 128x64, k 128, 2 stages
-    CFG_COUNT = 24
+    // int8 and fp32 (other forms: TILE_128): 128x128, 4 waves of 64x64, k-blocks
+    // direct to LDS through range-checked buffer loads (any layout, leading
+    // dimension and alignment; the grouped kernel's tile), three stages
+    CFG_GDMA_128 = 24,
+    CFG_COUNT = 25
 };
 
 const char* config_name(int cfg);

@@ -8,6 +8,7 @@
 #include "gemm_kernels.hpp"
 #include "launch.hpp"
 #include "wave_job.hpp"
+#include "grouped_dma.hpp"
 #include "occupancy.hpp"
 
 namespace ssb {
@@ -172,6 +173,30 @@ hipError_t run_direct(KParams p, hipStream_t st)
     return hipGetLastError();
 }
 
+// 128x128 tiles with k-blocks loaded straight into LDS through buffer
+// descriptors (int8, fp32; any layout, leading dimension and alignment).
+// Other forms, or matrices whose extent does not fit the descriptors' 31-bit
+// offsets: the register-staged 128x128 tile.
+template <class AR, bool AKC, bool BKC>
+hipError_t run_gdma(KParams p, hipStream_t st)
+{
+    if constexpr (AR::ID == ARITH_I8 || AR::ID == ARITH_F32) {
+        constexpr int E = AR::E;
+        const uint32_t abytes = AKC ? wave_extent(p.k, p.m, p.lda, E) : wave_extent(p.m, p.k, p.lda, E);
+        const uint32_t bbytes = BKC ? wave_extent(p.k, p.n, p.ldb, E) : wave_extent(p.n, p.k, p.ldb, E);
+        if (abytes != 0u && bbytes != 0u) {
+            p.tiles_m = (p.m + 127) / 128;
+            p.tiles_n = (p.n + 127) / 128;
+            p.total = (long)p.batch * p.tiles_m * p.tiles_n;
+            auto kern = gdma_gemm<AR, AKC, BKC>;
+            const long grid = persistent_grid(p.total, kernel_occupancy(kern, 256));
+            hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, p, abytes, bbytes);
+            return hipGetLastError();
+        }
+    }
+    return run_tile<AR, AKC, BKC, CFG_TILE_128>(p, st);
+}
+
 template <class AR, bool AKC, bool BKC>
 hipError_t run_cfg(const KParams& p, int cfg, hipStream_t st)
 {
@@ -200,6 +225,7 @@ hipError_t run_cfg(const KParams& p, int cfg, hipStream_t st)
     case CFG_DMA_256: return run_dma<AR, AKC, BKC, 256, 256, 64, 2, 4, 4>(p, st);
     case CFG_DMA_256_W16: return run_dma<AR, AKC, BKC, 256, 256, 64, 4, 4, 4>(p, st);
     case CFG_DMA_256_K2: return run_dma<AR, AKC, BKC, 256, 256, 128, 2, 4, 2>(p, st);
+    case CFG_GDMA_128: return run_gdma<AR, AKC, BKC>(p, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -29,7 +29,8 @@ SHAPES = [(1, 1, 1), (7, 5, 3), (16, 16, 64), (33, 17, 65), (64, 64, 64), (65, 6
           (130, 257, 200)]
 CONFIGS = ["direct16", "direct32", "tile64", "tile128", "tile128x64", "tile64x128", "tile128k2", "tile128w8",
            "tile256x128", "tile64s2", "tile128s2", "tile128k2s2", "tile128w8s2", "tile256x128s2", "dma64",
-           "dma128", "dma128k2", "dma256x128", "dma128s3", "kc256x128", "kc128", "dma256", "dma256w16", "dma256k2"]
+           "dma128", "dma128k2", "dma256x128", "dma128s3", "kc256x128", "kc128", "dma256", "dma256w16", "dma256k2",
+           "gdma128"]
 
 
 @pytest.fixture(autouse=True)
