@@ -88,9 +88,9 @@ struct GDmaOp<ArithI8, KC, MN, BK> {
     {
         return L::chunk_off(row, c, mn0, k0, ld);
     }
-    __device__ __forceinline__ static u32x4 frag(const char* lds, int fb, int ks, int lane)
+    __device__ __forceinline__ static void frag(const char* lds, int fb, int ks, int lane, u32x4* out)
     {
-        return L::frag(lds, fb, ks, lane);
+        out[0] = L::frag(lds, fb, ks, lane);
     }
 };
 
@@ -107,11 +107,9 @@ struct GDmaOp<ArithF32, true, MN, BK> {
     {
         return 4 * L::chunk_off(row, c, mn0, k0, ld);
     }
-    __device__ __forceinline__ static u32x4 frag(const char* lds, int fb, int ks, int lane)
+    __device__ __forceinline__ static void frag(const char* lds, int fb, int ks, int lane, u32x4* out)
     {
-        u32x4 out[1];
         L::frag(lds, fb + (lane & 15), ks, lane >> 4, out);
-        return out[0];
     }
 };
 
@@ -128,9 +126,66 @@ struct GDmaOp<ArithF32, false, MN, BK> {
     {
         return 4 * L::chunk_off(row, c, mn0, k0, ld);
     }
-    __device__ __forceinline__ static u32x4 frag(const char* lds, int fb, int ks, int lane)
+    __device__ __forceinline__ static void frag(const char* lds, int fb, int ks, int lane, u32x4* out)
     {
-        return L::frag(lds, fb, ks, lane);
+        out[0] = L::frag(lds, fb, ks, lane);
+    }
+};
+
+// The bf16 split forms keep fp32 in LDS in the fp32 layouts and cut each
+// fragment into its bf16 planes after reading it: a lane's eight k of one
+// 32-deep MFMA k-step are two 16-byte chunks (K-contiguous, KcOperand's
+// two-chunk read) or eight dwords from k-rows 32 ks + 8 g + i (M/N-contiguous).
+template <int NP, int ORD, int IDV, int MN, int BK>
+struct GDmaOp<ArithBF16Split<NP, ORD, IDV>, true, MN, BK> {
+    using AR = ArithBF16Split<NP, ORD, IDV>;
+    using L = KcOperand<AR, MN, BK>;
+    static constexpr int CP = L::CP, CHUNKS = L::CHUNKS, BYTES = L::BYTES, EPC = 4;
+    __device__ __forceinline__ static void coords(int s, int& row, int& c)
+    {
+        row = s / CP;
+        c = (s % CP) ^ dma_swz<CP>(row);
+    }
+    __device__ __forceinline__ static long off(int row, int c, int mn0, int k0, long ld)
+    {
+        return 4 * L::chunk_off(row, c, mn0, k0, ld);
+    }
+    __device__ __forceinline__ static void frag(const char* lds, int fb, int ks, int lane, u32x4* out)
+    {
+        L::frag(lds, fb + (lane & 15), ks, lane >> 4, out);
+    }
+};
+
+template <int NP, int ORD, int IDV, int MN, int BK>
+struct GDmaOp<ArithBF16Split<NP, ORD, IDV>, false, MN, BK> {
+    using AR = ArithBF16Split<NP, ORD, IDV>;
+    using L = F32MnOperand<MN, BK>;
+    static constexpr int CP = L::CP, CHUNKS = L::CHUNKS, BYTES = L::BYTES, EPC = 4;
+    static_assert(BK % 32 == 0, "whole bf16 k-steps");
+    __device__ __forceinline__ static void coords(int s, int& row, int& c)
+    {
+        row = s / CP;
+        c = (s % CP) ^ L::swz(row);
+    }
+    __device__ __forceinline__ static long off(int row, int c, int mn0, int k0, long ld)
+    {
+        return 4 * L::chunk_off(row, c, mn0, k0, ld);
+    }
+    __device__ __forceinline__ static void frag(const char* lds, int fb, int ks, int lane, u32x4* out)
+    {
+        const int mn = fb + (lane & 15), g = lane >> 4;
+        const int c = mn >> 2, w = mn & 3;
+        uint32_t v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int row = 32 * ks + 8 * g + i;
+            v[i] = *reinterpret_cast<const uint32_t*>(lds + L::slot(row, c) * 16 + 4 * w);
+        }
+        u32x4 lo[NP], hi[NP];
+        AR::to_planes(u32x4{v[0], v[1], v[2], v[3]}, lo);
+        AR::to_planes(u32x4{v[4], v[5], v[6], v[7]}, hi);
+#pragma unroll
+        for (int p = 0; p < NP; ++p) out[p] = u32x4{lo[p][0], lo[p][1], hi[p][0], hi[p][1]};
     }
 };
 
@@ -139,8 +194,10 @@ struct GDmaOp<ArithF32, false, MN, BK> {
 // three stages (48 KiB: three workgroups per CU).
 template <class AR, int T>
 struct GDmaCfg {
-    static constexpr int BK = (AR::ID == ARITH_I8 ? 64 : 16) * (128 / T);
-    static constexpr int S = 3;
+    static constexpr bool SPLIT = AR::NPLANE > 1;
+    // the bf16 forms' k-step is 32 deep: two stages of 32 KiB (two workgroups per CU)
+    static constexpr int BK = SPLIT ? 32 : (AR::ID == ARITH_I8 ? 64 : 16) * (128 / T);
+    static constexpr int S = SPLIT ? 2 : 3;
 };
 
 template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int S>
@@ -261,15 +318,15 @@ struct GDmaTile {
         const char* lb = la + OA::BYTES;
 #pragma unroll
         for (int ks = 0; ks < BK / AR::KSTEP; ++ks) {
-            u32x4 af[FM], bfg[FN];
+            u32x4 af[FM][AR::NPLANE], bfg[FN][AR::NPLANE];
 #pragma unroll
-            for (int fm = 0; fm < FM; ++fm) af[fm] = OA::frag(la, wm * WTM + fm * 16, ks, lane);
+            for (int fm = 0; fm < FM; ++fm) OA::frag(la, wm * WTM + fm * 16, ks, lane, af[fm]);
 #pragma unroll
-            for (int fn = 0; fn < FN; ++fn) bfg[fn] = OB::frag(lb, wn * WTN + fn * 16, ks, lane);
+            for (int fn = 0; fn < FN; ++fn) OB::frag(lb, wn * WTN + fn * 16, ks, lane, bfg[fn]);
 #pragma unroll
             for (int fm = 0; fm < FM; ++fm)
 #pragma unroll
-                for (int fn = 0; fn < FN; ++fn) AR::mma(acc[fm][fn], &af[fm], &bfg[fn]);
+                for (int fn = 0; fn < FN; ++fn) AR::mma(acc[fm][fn], af[fm], bfg[fn]);
         }
     }
 
@@ -393,12 +450,13 @@ __device__ __forceinline__ void g_run_dma(const TileJob& j, uint32_t abytes, uin
     }
 }
 
-// Batched kernel of 128x128 direct-to-LDS tiles (config gdma128): persistent
+// Batched kernel of 128x128 direct-to-LDS tiles (config gdma128; int8, fp32
+// and the bf16 split forms of fp32): persistent
 // workgroups walk the (instance, tile) work ids, neighbouring tiles of an
 // instance on one XCD.  abytes / bbytes: extent of every instance's stored A
 // / B.
 template <class AR, bool AKC, bool BKC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3)))
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR::NPLANE > 1 ? 2 : 3)))
 gdma_gemm(const KParams p, uint32_t abytes, uint32_t bbytes)
 {
     __shared__ __attribute__((aligned(1024))) char smem[gdma_lds_bytes<AR, 128>()];

@@ -174,13 +174,14 @@ hipError_t run_direct(KParams p, hipStream_t st)
 }
 
 // 128x128 tiles with k-blocks loaded straight into LDS through buffer
-// descriptors (int8, fp32; any layout, leading dimension and alignment).
+// descriptors (int8, fp32 and its bf16 split forms; any layout, leading
+// dimension and alignment).
 // Other forms, or matrices whose extent does not fit the descriptors' 31-bit
 // offsets: the register-staged 128x128 tile.
 template <class AR, bool AKC, bool BKC>
 hipError_t run_gdma(KParams p, hipStream_t st)
 {
-    if constexpr (AR::ID == ARITH_I8 || AR::ID == ARITH_F32) {
+    if constexpr (AR::ID == ARITH_I8 || AR::ID == ARITH_F32 || AR::NPLANE > 1) {
         constexpr int E = AR::E;
         const uint32_t abytes = AKC ? wave_extent(p.k, p.m, p.lda, E) : wave_extent(p.m, p.k, p.lda, E);
         const uint32_t bbytes = BKC ? wave_extent(p.k, p.n, p.ldb, E) : wave_extent(p.n, p.k, p.ldb, E);

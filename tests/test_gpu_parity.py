@@ -208,7 +208,7 @@ def test_kc_dma_tiles(cfg, form, ct, factor, grid):
 SPLIT_FORMS = [("bf16x6", 5, 4.0), ("bf16x3", 4, None)]
 
 
-@pytest.mark.parametrize("cfg", ["tile256x128", "tile256x128s2", "tile128w8", "tile128", "tile64"])
+@pytest.mark.parametrize("cfg", ["tile256x128", "tile256x128s2", "tile128w8", "tile128", "tile64", "gdma128"])
 @pytest.mark.parametrize("form,ct,factor", SPLIT_FORMS, ids=[f[0] for f in SPLIT_FORMS])
 @pytest.mark.parametrize("ta,tb", OPS)
 def test_split_forms_forced_tiles(cfg, form, ct, factor, ta, tb):
