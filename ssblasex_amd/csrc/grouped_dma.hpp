@@ -414,19 +414,7 @@ __device__ __forceinline__ void gdma_job(const TileJob& j, uint32_t abytes, uint
 
     Out* C = static_cast<Out*>(j.C);
     const int row0 = j.i0 + wm * T::WTM + 4 * (lane >> 4), col0 = j.j0 + wn * T::WTN + (lane & 15);
-    if (j.beta0 != 0) {
-#pragma unroll
-        for (int fm = 0; fm < T::FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < T::FN; ++fn)
-                gstore_frag<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, true, j.m, j.n);
-    } else {
-#pragma unroll
-        for (int fm = 0; fm < T::FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < T::FN; ++fn)
-                gstore_frag<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, false, j.m, j.n);
-    }
+    gstore_block<AR, T::FM, T::FN>(C, j.ldc, row0, col0, acc, j.s, j.beta0 != 0, j.m, j.n);
 }
 
 template <class AR, int TS>

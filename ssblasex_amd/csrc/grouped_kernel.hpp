@@ -286,24 +286,9 @@ __device__ __forceinline__ void gtile_job(const TileJob& j, uint32_t abytes, uin
         __syncthreads();
     }
 
-    // beta == 0 decided once, outside the fragment loops: a conditional
-    // load of old C inside them would put a vmcnt wait (which also waits
-    // for the earlier stores) in front of every store
     Out* C = static_cast<Out*>(j.C);
     const int row0 = j.i0 + wm * S::WTM + 4 * (lane >> 4), col0 = j.j0 + wn * S::WTN + (lane & 15);
-    if (j.beta0 != 0) {
-#pragma unroll
-        for (int fm = 0; fm < S::FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < S::FN; ++fn)
-                gstore_frag<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, true, j.m, j.n);
-    } else {
-#pragma unroll
-        for (int fm = 0; fm < S::FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < S::FN; ++fn)
-                gstore_frag<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, false, j.m, j.n);
-    }
+    gstore_block<AR, S::FM, S::FN>(C, j.ldc, row0, col0, acc, j.s, j.beta0 != 0, j.m, j.n);
 }
 
 template <class AR, int CFG>

@@ -73,6 +73,32 @@ __device__ __forceinline__ void gstore_frag(typename AR::Out* C, long ldc, int r
 }
 
 
+// A wave's FM x FN fragments stored column block by column block: the two
+// 16-row halves of a column's 128-byte line (fragments 2q and 2q + 1 of one
+// column block) leave back to back, so the L2 writes the line once.  beta == 0
+// decided once, outside the loops: a conditional load of old C inside them
+// would put a vmcnt wait (which also waits for the earlier stores) in front of
+// every store.
+template <class AR, int FM, int FN>
+__device__ __forceinline__ void gstore_block(typename AR::Out* C, long ldc, int row0, int col0,
+                                             const typename AR::Acc (&acc)[FM][FN], const Scalars& sc, bool beta0,
+                                             int m, int n)
+{
+    if (beta0) {
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm)
+                gstore_frag<AR>(C, ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], sc, true, m, n);
+    } else {
+#pragma unroll
+        for (int fn = 0; fn < FN; ++fn)
+#pragma unroll
+            for (int fm = 0; fm < FM; ++fm)
+                gstore_frag<AR>(C, ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], sc, false, m, n);
+    }
+}
+
 // The old C values of one fragment (beta != 0) at any ldc and alignment,
 // zero outside the matrix.
 template <class AR>
@@ -323,26 +349,18 @@ __device__ __forceinline__ void wave_job(const TileJob& j, uint32_t abytes, uint
         run(a1, b1, s + 1);
     }
 
-    if (j.beta0 != 0) {
-#pragma unroll
-        for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < FN; ++fn)
-                gstore_frag<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, true, j.m, j.n);
-    } else if constexpr (PREFETCH_C) {
-#pragma unroll
-        for (int fm = 0; fm < FM; ++fm)
+    if constexpr (PREFETCH_C) {
+        if (j.beta0 == 0) {
 #pragma unroll
             for (int fn = 0; fn < FN; ++fn)
-                gstore_frag_old<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, j.m, j.n,
-                                    old[fm][fn]);
-    } else {
 #pragma unroll
-        for (int fm = 0; fm < FM; ++fm)
-#pragma unroll
-            for (int fn = 0; fn < FN; ++fn)
-                gstore_frag<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, false, j.m, j.n);
+                for (int fm = 0; fm < FM; ++fm)
+                    gstore_frag_old<AR>(C, j.ldc, row0 + fm * 16, col0 + fn * 16, acc[fm][fn], j.s, j.m, j.n,
+                                        old[fm][fn]);
+            return;
+        }
     }
+    gstore_block<AR, FM, FN>(C, j.ldc, row0, col0, acc, j.s, j.beta0 != 0, j.m, j.n);
 }
 
 // Bytes a buffer descriptor over a stored matrix spans (rows x cols at
