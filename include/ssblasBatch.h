@@ -213,7 +213,9 @@ ssblasStatus_t ssblasGroupedPlanCreate(
     ssblasComputeType_t      computeType,
     ssblasGemmAlgo_t         algo);
 
-/* Enqueue the plan's single persistent launch on `stream` (capturable). */
+/* Enqueue the plan's single persistent launch on `stream` (capturable).
+ * Launches of one plan must not overlap in time: workgroups claim the plan's
+ * work through counters held in the plan (zero again when a launch ends). */
 ssblasStatus_t ssblasGroupedPlanExecute(ssblasGroupedPlan_t plan, ssblasStream_t stream);
 
 /* First call captures ssblasGroupedPlanExecute into a HIP graph owned by the
