@@ -20,6 +20,7 @@
 // Included by one .hip file per family.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 
 #include "gemm_kernels.hpp"
 #include "grouped.hpp"
@@ -605,7 +606,13 @@ hipError_t run_grouped(const GroupedDevice& d, hipStream_t st)
 {
     if (d.nitems <= 0) return hipSuccess;
     auto kern = grouped_gemm<FA, FB>;
-    const long slots = (long)kernel_occupancy(kern, 256) * device_cu_count();
+    long slots = (long)kernel_occupancy(kern, 256) * device_cu_count();
+    // SSBLAS_MAX_GRID=<n> caps the grid (tests: few workgroups, some queues
+    // with no workgroup of their own, every entry past the first claimed)
+    if (const char* e = getenv("SSBLAS_MAX_GRID")) {
+        const long cap = atol(e);
+        if (cap >= 1 && cap < slots) slots = cap;
+    }
     const long want = (long)d.nitems;
     const long grid = want < slots ? want : slots;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, d);

@@ -205,6 +205,30 @@ def test_grouped_graph_replays(cfg, monkeypatch):
             check(G, dg.results())
 
 
+@pytest.mark.parametrize("grid", ["1", "3", "13"])
+def test_grouped_capped_grid(grid, monkeypatch):
+    """A grid of 1, 3 or 13 workgroups: queues without a workgroup of their
+    own are drained by workgroups of other XCDs, every workgroup claims many
+    entries (tiles and bundles of one-wave items, both forms); the plan is
+    executed twice in a row and then replayed from its graph, each launch
+    starting from the counters the previous one re-armed."""
+    import torch
+
+    monkeypatch.setenv("SSBLAS_MAX_GRID", grid)
+    dims = [(300, 200, 150), (40, 300, 70), (20, 20, 500), (129, 257, 33), (8, 9, 10), (64, 64, 64),
+            (200, 31, 300), (128, 128, 1)]
+    G = make_grouped(len(dims) * 6, seed=0x55B1A5 + 59, group=4, dims=dims * 6)
+    dg = DeviceGrouped(G)
+    with plan_for(dg) as p:
+        for launch in ("execute", "execute", "graph"):
+            dg.reset_c()
+            if launch == "graph":
+                assert p.graph_launch(torch.cuda.current_stream()) == 0
+            else:
+                assert p.execute() == 0
+            check(G, dg.results())
+
+
 def test_grouped_one_shot_entry():
     import ssblasex_amd as S
 
