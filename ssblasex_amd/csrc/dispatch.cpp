@@ -38,6 +38,7 @@ const char* config_name(int cfg)
     case CFG_DMA_256_W16: return "dma256w16";
     case CFG_DMA_256_K2: return "dma256k2";
     case CFG_GDMA_128: return "gdma128";
+    case CFG_GDMA_64: return "gdma64";
     default: return "unknown";
     }
 }
@@ -49,7 +50,7 @@ void config_tile(int cfg, int* tm, int* tn)
                                            {128, 128}, {128, 128}, {128, 128}, {256, 128}, {64, 64},
                                            {128, 128}, {128, 128}, {256, 128}, {128, 128},
                                            {256, 128}, {128, 128}, {256, 256}, {256, 256}, {256, 256},
-                                           {128, 128}};
+                                           {128, 128}, {64, 64}};
     if (cfg < 0 || cfg >= CFG_COUNT) cfg = CFG_TILE_64;
     *tm = dims[cfg][0];
     *tn = dims[cfg][1];

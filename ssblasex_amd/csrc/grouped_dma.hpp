@@ -443,18 +443,18 @@ __device__ __forceinline__ void g_run_dma(const TileJob& j, uint32_t abytes, uin
 // workgroups walk the (instance, tile) work ids, neighbouring tiles of an
 // instance on one XCD.  abytes / bbytes: extent of every instance's stored A
 // / B.
-template <class AR, bool AKC, bool BKC>
+template <class AR, bool AKC, bool BKC, int TS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR::NPLANE > 1 ? 2 : 3)))
 gdma_gemm(const KParams p, uint32_t abytes, uint32_t bbytes)
 {
-    __shared__ __attribute__((aligned(1024))) char smem[gdma_lds_bytes<AR, 128>()];
+    __shared__ __attribute__((aligned(1024))) char smem[gdma_lds_bytes<AR, TS>()];
     const int tiles_per_inst = p.tiles_m * p.tiles_n;
     for (long t = blockIdx.x; t < p.total; t += gridDim.x) {
         const long w = xcd_remap(t, p.total);
         const int inst = (int)(w / tiles_per_inst);
         const int rem = (int)(w % tiles_per_inst);
-        const TileJob j = make_job(p, inst, (rem % p.tiles_m) * 128, (rem / p.tiles_m) * 128);
-        gdma_job<AR, 128, AKC, BKC>(j, abytes, bbytes, smem, [] {}, [] {});
+        const TileJob j = make_job(p, inst, (rem % p.tiles_m) * TS, (rem / p.tiles_m) * TS);
+        gdma_job<AR, TS, AKC, BKC>(j, abytes, bbytes, smem, [] {}, [] {});
     }
 }
 

@@ -53,7 +53,8 @@ enum TileConfig : int {
     // direct to LDS through range-checked buffer loads (any layout, leading
     // dimension and alignment; the grouped kernel's tile), three stages
     CFG_GDMA_128 = 24,
-    CFG_COUNT = 25
+    CFG_GDMA_64 = 25,  // the same with 64x64 tiles (four waves of 32x32; other forms: TILE_64)
+    CFG_COUNT = 26
 };
 
 const char* config_name(int cfg);
