@@ -1,5 +1,6 @@
-// grouped_host.cpp -- grouped plans: argument checks, the tile shape of each
-// instance, the cost-sorted device work list, execution and graph capture.
+// grouped_host.cpp -- grouped plans: argument checks, the tile shapes of
+// each instance, the device list of entries in per-XCD queues with their
+// claim counters, execution and graph capture.
 #include <limits.h>
 #include <stdlib.h>
 #include <string.h>
@@ -14,7 +15,7 @@
 
 struct ssblasGroupedPlan {
     int device = 0;
-    void* mem = nullptr;  // groups, instances, items: one device allocation
+    void* mem = nullptr;  // entries and claim counters: one device allocation
     int nfam = 0;
     int fam_id[ssb::GF_COUNT] = {};
     ssb::GroupedDevice fam[ssb::GF_COUNT] = {};

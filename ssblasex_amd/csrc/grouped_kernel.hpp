@@ -1,21 +1,24 @@
 // grouped_kernel.hpp -- the grouped (variable-size, mixed-type) batched GEMM:
-// one persistent kernel per family walks a device work list built on the
-// host (grouped_host.cpp).
+// one persistent kernel per family walks a device list of entries built on
+// the host (grouped_host.cpp).
 //
-// Work item = one output tile of one instance: the tile shape chosen for the
-// instance (32x32, 64x64 or 128x128), and through the instance its group's
-// arithmetic form, operand layouts and scalars.  Items sit in per-XCD
-// queues sorted by estimated cost, longest first, and every workgroup takes
-// a fixed share of its queue in turn with the others, so long tiles start
-// first and short ones fill in behind.  A launch writes nothing but C (a
-// captured graph replays it as is).
+// Entry = one output tile of one instance (32x32, 64x64 or 128x128: the
+// shape chosen for the instance, or a smaller one for the rows / columns
+// left over past its last whole tile), or a bundle of four one-wave 32x32
+// items; through the instance come its group's arithmetic form, operand
+// layouts and scalars.  Entries sit in per-XCD queues, instance by instance,
+// and workgroups claim them (below), so long tiles start first and short
+// ones fill in behind.  A launch writes C and the plan's claim counters,
+// which its last workgroup returns to zero (a captured graph replays it as
+// is).
 //
 // Operands are read with buffer loads through a descriptor that spans the
 // stored matrix: whole 16-byte (or 4-byte) words at any leading dimension
 // and alignment, where a dword past the end of the matrix reads as zero
 // instead of faulting.  Partial tiles take the same loads as whole ones;
 // only a k tail (and, for int8, the word holding a matrix's last bytes) is
-// fixed in registers before the LDS store.
+// fixed, in registers (32x32 tiles, one-wave items) or in LDS (the
+// direct-to-LDS 64x64 and 128x128 tiles of grouped_dma.hpp).
 //
 // Included by one .hip file per family.
 #pragma once
@@ -209,6 +212,8 @@ struct GStage {
 // One BM x BN tile of one grouped instance, 4 waves (2 x 2), LDS
 // double-buffered, global loads of k-block kt+1 in flight while kt is
 // multiplied.  abytes / bbytes: extent of the stored A / B in bytes.
+// `started` runs once the first k-block's loads are issued, `first_block`
+// once that block is in LDS (the entry loop's claim and prefetch hooks).
 template <class AR, int BM, int BN, int BK, class Started, class Hook>
 __device__ __forceinline__ void gtile_job(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
                                           Started&& started, Hook&& first_block)
@@ -350,11 +355,12 @@ __device__ __forceinline__ void g_run(int cfg, const TileJob& j, uint32_t abytes
     }
 }
 
-// k-steps per stage of a wave item
+// k-steps per stage of a one-wave item
 constexpr int G_WAVE_NS = 2;
 
-// One wave's 32x32 block of a grouped instance (W32 items), operand layouts
-// chosen at run time.  slot: 4 KiB of LDS private to the wave.
+// One wave's 32x32 block of a grouped instance (a one-wave item of a
+// bundle), operand layouts chosen at run time.  slot: 4 KiB of LDS private
+// to the wave.
 template <class AR>
 __device__ __forceinline__ void g_run_wave(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* slot)
 {
