@@ -41,15 +41,18 @@ namespace ssb {
 // i < 4, of tile row fb + r (the k assignment of KcOperand's chunks), four
 // dword reads from rows that share their swizzle; the swizzle moves the four
 // lane groups' rows (k differing in bits 2..3) to different 64-byte bank
-// ranges.
+// ranges (32 columns, 128-byte rows: the two lane groups of each half-wave).
 template <int MN, int BK>
 struct F32MnOperand {
     static constexpr int CP = MN / 4;
     static constexpr int CHUNKS = BK * CP;
     static constexpr int BYTES = CHUNKS * 16;
-    static_assert(CP >= 16 && BK % 16 == 0, "four 64-byte column groups, whole k-steps");
+    static_assert((CP == 8 || CP >= 16) && BK % 16 == 0, "two or four 64-byte column groups, whole k-steps");
 
-    __device__ __forceinline__ static int swz(int row) { return ((row >> 2) & 3) << 2; }
+    __device__ __forceinline__ static int swz(int row)
+    {
+        return CP >= 16 ? ((row >> 2) & 3) << 2 : ((row >> 2) & 1) << 2;
+    }
     __device__ __forceinline__ static int slot(int row, int c) { return row * CP + (c ^ swz(row)); }
     __device__ __forceinline__ static long chunk_off(int row, int c, int mn0, int k0, long ld)
     {
@@ -200,12 +203,13 @@ struct GDmaCfg {
     static constexpr int S = SPLIT ? 2 : 3;
 };
 
-template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int S>
+template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int S, int NW = 4>
 struct GDmaTile {
     using OA = GDmaOp<AR, AKC, BM, BK>;
     using OB = GDmaOp<AR, BKC, BN, BK>;
-    static constexpr int NWAVE = 4;                      // 2 x 2 waves
-    static constexpr int WTM = BM / 2, WTN = BN / 2;
+    static_assert(NW == 4 || NW == 1, "2 x 2 waves, or one wave over the whole tile");
+    static constexpr int NWAVE = NW;
+    static constexpr int WTM = NW == 4 ? BM / 2 : BM, WTN = NW == 4 ? BN / 2 : BN;
     static constexpr int FM = WTM / 16, FN = WTN / 16;
     static constexpr int STAGE = OA::BYTES + OB::BYTES;
     static constexpr int LDS_BYTES = S * STAGE;
@@ -415,6 +419,79 @@ __device__ __forceinline__ void gdma_job(const TileJob& j, uint32_t abytes, uint
     Out* C = static_cast<Out*>(j.C);
     const int row0 = j.i0 + wm * T::WTM + 4 * (lane >> 4), col0 = j.j0 + wn * T::WTN + (lane & 15);
     gstore_block<AR, T::FM, T::FN>(C, j.ldc, row0, col0, acc, j.s, j.beta0 != 0, j.m, j.n);
+}
+
+// One wave's 32x32 block of a grouped int8 / fp32 instance (a one-wave item
+// of a bundle) with the same direct-to-LDS loads: k-blocks of 4 KiB (int8
+// 32x64 and 64x32, fp32 32x16 and 16x32), three stages in the wave's own
+// 12 KiB of LDS, so two k-blocks are in flight while one is multiplied and
+// no staging registers are held.  Only the wave reads its LDS: it waits for
+// its own loads and needs no barrier.  `started` runs once the first
+// k-blocks' loads are issued.
+constexpr int G_WAVE_LDS = 12288;
+
+template <class AR>
+struct GWaveCfg {
+    static constexpr int BK = AR::ID == ARITH_I8 ? 64 : 16, S = 3;
+};
+
+template <class AR, bool AKC, bool BKC, class Started>
+__device__ __forceinline__ void gdma_wave_job(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* slot,
+                                              Started&& started)
+{
+    constexpr int BK = GWaveCfg<AR>::BK, S = GWaveCfg<AR>::S, TS = 32;
+    using T = GDmaTile<AR, AKC, BKC, TS, TS, BK, S, 1>;
+    static_assert(T::LDS_BYTES <= G_WAVE_LDS, "three stages in the wave's LDS");
+    using Out = typename AR::Out;
+    const int lane = threadIdx.x & 63;
+    const int nk = (j.k + BK - 1) / BK;
+    const __amdgpu_buffer_rsrc_t ra = make_rsrc(j.A, abytes);
+    const __amdgpu_buffer_rsrc_t rb = make_rsrc(j.B, bbytes);
+    constexpr int E = AR::E;
+    const long sra = AKC ? j.lda : 1, ska = AKC ? 1 : j.lda, srb = BKC ? j.ldb : 1, skb = BKC ? 1 : j.ldb;
+    const uint32_t a_org = (uint32_t)((long)j.i0 * sra * E), b_org = (uint32_t)((long)j.j0 * srb * E);
+    const uint32_t a_kstep = (uint32_t)(BK * ska * E), b_kstep = (uint32_t)(BK * skb * E);
+    const uint32_t a_span = (uint32_t)(((TS - 1) * sra + (BK - 1) * ska + 1) * E);
+    const uint32_t b_span = (uint32_t)(((TS - 1) * srb + (BK - 1) * skb + 1) * E);
+
+    typename AR::Acc acc[T::FM][T::FN];
+#pragma unroll
+    for (int fm = 0; fm < T::FM; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < T::FN; ++fn) acc[fm][fn] = typename AR::Acc{};
+
+    __asm__ __volatile__("" ::: "memory");
+#pragma unroll
+    for (int s = 0; s < S - 1; ++s)
+        if (s < nk) T::issue(j, ra, rb, s, slot + s * T::STAGE, 0, lane);
+    started();
+    for (int kt = 0; kt < nk; ++kt) {
+        char* cur = slot + (kt % S) * T::STAGE;
+        T::wait_block(min(nk - 1 - kt, S - 2));
+        const int k0 = kt * BK, kv = j.k - k0;
+        if constexpr (AR::E == 1) {
+            if (a_org + (uint32_t)kt * a_kstep + a_span > abytes)
+                T::template fix_end<typename T::OA, T::A_PW>(cur, ra, abytes, j.i0, k0, j.lda, 0, lane);
+            if (b_org + (uint32_t)kt * b_kstep + b_span > bbytes)
+                T::template fix_end<typename T::OB, T::B_PW>(cur + T::OA::BYTES, rb, bbytes, j.j0, k0, j.ldb, 0,
+                                                             lane);
+        }
+        if (kv < BK) {
+            if constexpr (AKC) T::template clear_k<typename T::OA, T::A_PW>(cur, kv, 0, lane);
+            if constexpr (BKC) T::template clear_k<typename T::OB, T::B_PW>(cur + T::OA::BYTES, kv, 0, lane);
+        }
+        // the fixes and the previous block's fragment reads have completed
+        // before the block is read and its stage refilled
+        wait_lgkm0();
+        __asm__ __volatile__("" ::: "memory");
+        if (kt + S - 1 < nk) T::issue(j, ra, rb, kt + S - 1, slot + ((kt + S - 1) % S) * T::STAGE, 0, lane);
+        T::compute(acc, cur, 0, 0, lane);
+    }
+    __asm__ __volatile__("" ::: "memory");
+
+    Out* C = static_cast<Out*>(j.C);
+    gstore_block<AR, T::FM, T::FN>(C, j.ldc, j.i0 + 4 * (lane >> 4), j.j0 + (lane & 15), acc, j.s, j.beta0 != 0,
+                                   j.m, j.n);
 }
 
 template <class AR, int TS>

@@ -17,8 +17,9 @@
 // and alignment, where a dword past the end of the matrix reads as zero
 // instead of faulting.  Partial tiles take the same loads as whole ones;
 // only a k tail (and, for int8, the word holding a matrix's last bytes) is
-// fixed, in registers (32x32 tiles, one-wave items) or in LDS (the
-// direct-to-LDS 64x64 and 128x128 tiles of grouped_dma.hpp).
+// fixed, in registers (32x32 tiles; the fp64 and bf16 one-wave items) or in
+// LDS (the direct-to-LDS int8 / fp32 tiles and one-wave items of
+// grouped_dma.hpp).
 //
 // Included by one .hip file per family.
 #pragma once
@@ -324,7 +325,10 @@ constexpr int g_lds_max()
 {
     constexpr int a = g_tile_lds<AR, GCFG_32>(), b = g_tile_lds<AR, GCFG_64>(), c = g_tile_lds<AR, GCFG_128>();
     constexpr int ab = a > b ? a : b;
-    return grouped_max_cfg(AR::ID) == GCFG_128 ? (ab > c ? ab : c) : ab;
+    constexpr int t = grouped_max_cfg(AR::ID) == GCFG_128 ? (ab > c ? ab : c) : ab;
+    // four waves' one-wave items side by side
+    constexpr int w = AR::ID == ARITH_I8 || AR::ID == ARITH_F32 ? 4 * G_WAVE_LDS : 0;
+    return t > w ? t : w;
 }
 
 template <class AR, class Started, class Hook>
@@ -355,25 +359,25 @@ __device__ __forceinline__ void g_run(int cfg, const TileJob& j, uint32_t abytes
     }
 }
 
-// k-steps per stage of a one-wave item
-constexpr int G_WAVE_NS = 2;
-
 // One wave's 32x32 block of a grouped instance (a one-wave item of a
-// bundle), operand layouts chosen at run time.  slot: 4 KiB of LDS private
-// to the wave.
-template <class AR>
-__device__ __forceinline__ void g_run_wave(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* slot)
+// bundle), operand layouts chosen at run time: int8 and fp32 through the
+// wave's direct-to-LDS job in `slot` (G_WAVE_LDS bytes of LDS private to the
+// wave), fp64 and the bf16 split forms with operands loaded straight into
+// registers.  `started` runs once the item's first loads are issued.
+template <class AR, class Started>
+__device__ __forceinline__ void g_run_wave(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* slot,
+                                           Started&& started)
 {
     if constexpr (AR::ID == ARITH_I8 || AR::ID == ARITH_F32) {
-        constexpr int NS = G_WAVE_NS;
         if (j.akc) {
-            if (j.bkc) wave_job<AR, true, true, 2, 2, NS, false>(j, abytes, bbytes, slot);
-            else wave_job<AR, true, false, 2, 2, NS, false>(j, abytes, bbytes, slot);
+            if (j.bkc) gdma_wave_job<AR, true, true>(j, abytes, bbytes, slot, started);
+            else gdma_wave_job<AR, true, false>(j, abytes, bbytes, slot, started);
         } else {
-            if (j.bkc) wave_job<AR, false, true, 2, 2, NS, false>(j, abytes, bbytes, slot);
-            else wave_job<AR, false, false, 2, 2, NS, false>(j, abytes, bbytes, slot);
+            if (j.bkc) gdma_wave_job<AR, false, true>(j, abytes, bbytes, slot, started);
+            else gdma_wave_job<AR, false, false>(j, abytes, bbytes, slot, started);
         }
     } else {
+        started();
         if (j.akc) {
             if (j.bkc) direct_job<AR, true, true, 32, 32>(j);
             else direct_job<AR, true, false, 32, 32>(j);
@@ -519,7 +523,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(g_wave
     constexpr int LA = g_lds_max<FA>(), LB = g_lds_max<FB>();
     __shared__ __attribute__((aligned(16))) char smem[LA > LB ? LA : LB];
     __shared__ int s_claim[3];
-    static_assert((LA > LB ? LA : LB) >= 4 * 4096, "a 4 KiB LDS slot per wave for the one-wave items");
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const bool leader = threadIdx.x == 0;
@@ -570,11 +573,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(g_wave
                 if (leader) s_claim[slot] = g_claim_finish(d.qoff, qctr, G, cs, cl);
             };
             if (w.cfg == GCFG_W32) {
-                started();
                 if (w.j.m > 0) {
-                    char* slotp = smem + 4096 * wave;
-                    if (w.form == FA::ID) g_run_wave<FA>(w.j, w.abytes, w.bbytes, slotp);
-                    else g_run_wave<FB>(w.j, w.abytes, w.bbytes, slotp);
+                    char* slotp = smem + G_WAVE_LDS * wave;
+                    if (w.form == FA::ID) g_run_wave<FA>(w.j, w.abytes, w.bbytes, slotp, started);
+                    else g_run_wave<FB>(w.j, w.abytes, w.bbytes, slotp, started);
+                } else {
+                    started();
                 }
                 loaded();
             } else {
