@@ -1,5 +1,6 @@
 // grouped_dma.hpp -- the grouped kernel's 64x64 and 128x128 workgroup tiles
-// for the int8 and fp32 forms with direct-to-LDS buffer loads.
+// and its one-wave 32x32 items for the int8 and fp32 forms with
+// direct-to-LDS buffer loads.
 //
 // Every k-block of A and B moves global -> LDS with buffer_load_dwordx4 ...
 // lds (64 lanes x 16 bytes -> 1 KiB of LDS per wave instruction, no staging
@@ -18,7 +19,8 @@
 //
 // What the loads cannot get right by themselves, fixed in LDS by the lane
 // that loaded the chunk, after its wave's loads of the block have landed and
-// before the barrier that publishes the block:
+// before the barrier that publishes the block (a one-wave item: before the
+// wave reads the block):
 //  * the k tail of a K-contiguous operand (last k-block): elements at k >= K
 //    hold the next row or padding and are cleared (M/N-contiguous chunks at
 //    k >= K start past the end of the matrix and are zero already);
