@@ -39,6 +39,7 @@ const char* config_name(int cfg)
     case CFG_DMA_256_K2: return "dma256k2";
     case CFG_GDMA_128: return "gdma128";
     case CFG_GDMA_64: return "gdma64";
+    case CFG_GDMA_256x128: return "gdma256x128";
     default: return "unknown";
     }
 }
@@ -50,7 +51,7 @@ void config_tile(int cfg, int* tm, int* tn)
                                            {128, 128}, {128, 128}, {128, 128}, {256, 128}, {64, 64},
                                            {128, 128}, {128, 128}, {256, 128}, {128, 128},
                                            {256, 128}, {128, 128}, {256, 256}, {256, 256}, {256, 256},
-                                           {128, 128}, {64, 64}};
+                                           {128, 128}, {64, 64},   {256, 128}};
     if (cfg < 0 || cfg >= CFG_COUNT) cfg = CFG_TILE_64;
     *tm = dims[cfg][0];
     *tn = dims[cfg][1];

@@ -209,9 +209,10 @@ template <class AR, bool AKC, bool BKC, int BM, int BN, int BK, int S, int NW = 
 struct GDmaTile {
     using OA = GDmaOp<AR, AKC, BM, BK>;
     using OB = GDmaOp<AR, BKC, BN, BK>;
-    static_assert(NW == 4 || NW == 1, "2 x 2 waves, or one wave over the whole tile");
+    static_assert(NW == 8 || NW == 4 || NW == 1, "4 x 2 or 2 x 2 waves, or one wave over the whole tile");
     static constexpr int NWAVE = NW;
-    static constexpr int WTM = NW == 4 ? BM / 2 : BM, WTN = NW == 4 ? BN / 2 : BN;
+    static constexpr int WAVES_M = NW == 8 ? 4 : NW == 4 ? 2 : 1, WAVES_N = NW == 1 ? 1 : 2;
+    static constexpr int WTM = BM / WAVES_M, WTN = BN / WAVES_N;
     static constexpr int FM = WTM / 16, FN = WTN / 16;
     static constexpr int STAGE = OA::BYTES + OB::BYTES;
     static constexpr int LDS_BYTES = S * STAGE;
@@ -355,16 +356,16 @@ struct GDmaTile {
 // tile's loads have all landed (the item loop settles the next
 // item's prefetched description there).  Starts with a barrier: the
 // previous item's LDS readers are done.
-template <class AR, int TS, bool AKC, bool BKC, class Started, class Hook>
+template <class AR, int TS, bool AKC, bool BKC, int BM = TS, int NW = 4, class Started, class Hook>
 __device__ __forceinline__ void gdma_job(const TileJob& j, uint32_t abytes, uint32_t bbytes, char* smem,
                                          Started&& started, Hook&& done_loading)
 {
     constexpr int BK = GDmaCfg<AR, TS>::BK, S = GDmaCfg<AR, TS>::S;
-    using T = GDmaTile<AR, AKC, BKC, TS, TS, BK, S>;
+    using T = GDmaTile<AR, AKC, BKC, BM, TS, BK, S, NW>;
     using Out = typename AR::Out;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % T::WAVES_M, wn = wave / T::WAVES_M;
     const int nk = (j.k + BK - 1) / BK;
     const __amdgpu_buffer_rsrc_t ra = make_rsrc(j.A, abytes);
     const __amdgpu_buffer_rsrc_t rb = make_rsrc(j.B, bbytes);
@@ -374,7 +375,7 @@ __device__ __forceinline__ void gdma_job(const TileJob& j, uint32_t abytes, uint
     const long sra = AKC ? j.lda : 1, ska = AKC ? 1 : j.lda, srb = BKC ? j.ldb : 1, skb = BKC ? 1 : j.ldb;
     const uint32_t a_org = (uint32_t)((long)j.i0 * sra * E), b_org = (uint32_t)((long)j.j0 * srb * E);
     const uint32_t a_kstep = (uint32_t)(BK * ska * E), b_kstep = (uint32_t)(BK * skb * E);
-    const uint32_t a_span = (uint32_t)(((TS - 1) * sra + (BK - 1) * ska + 1) * E);
+    const uint32_t a_span = (uint32_t)(((BM - 1) * sra + (BK - 1) * ska + 1) * E);
     const uint32_t b_span = (uint32_t)(((TS - 1) * srb + (BK - 1) * skb + 1) * E);
 
     typename AR::Acc acc[T::FM][T::FN];
@@ -496,11 +497,11 @@ __device__ __forceinline__ void gdma_wave_job(const TileJob& j, uint32_t abytes,
                                    j.m, j.n);
 }
 
-template <class AR, int TS>
+template <class AR, int TS, int BM = TS, int NW = 4>
 constexpr int gdma_lds_bytes()
 {
-    using T = GDmaTile<AR, true, true, TS, TS, GDmaCfg<AR, TS>::BK, GDmaCfg<AR, TS>::S>;
-    using U = GDmaTile<AR, false, false, TS, TS, GDmaCfg<AR, TS>::BK, GDmaCfg<AR, TS>::S>;
+    using T = GDmaTile<AR, true, true, BM, TS, GDmaCfg<AR, TS>::BK, GDmaCfg<AR, TS>::S, NW>;
+    using U = GDmaTile<AR, false, false, BM, TS, GDmaCfg<AR, TS>::BK, GDmaCfg<AR, TS>::S, NW>;
     return T::LDS_BYTES > U::LDS_BYTES ? T::LDS_BYTES : U::LDS_BYTES;
 }
 
@@ -517,23 +518,24 @@ __device__ __forceinline__ void g_run_dma(const TileJob& j, uint32_t abytes, uin
     }
 }
 
-// Batched kernel of 128x128 direct-to-LDS tiles (config gdma128; int8, fp32
-// and the bf16 split forms of fp32): persistent
-// workgroups walk the (instance, tile) work ids, neighbouring tiles of an
-// instance on one XCD.  abytes / bbytes: extent of every instance's stored A
+// Batched kernel of BM x TS direct-to-LDS tiles of NW waves (configs
+// gdma256x128: 8 waves of 64x64; gdma128, gdma64: 4 waves; int8, fp32 and the
+// bf16 split forms of fp32): persistent workgroups walk the (instance, tile)
+// work ids, neighbouring tiles of an instance on one XCD.  abytes / bbytes: extent of every instance's stored A
 // / B.
-template <class AR, bool AKC, bool BKC, int TS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(AR::NPLANE > 1 ? 2 : 3)))
-gdma_gemm(const KParams p, uint32_t abytes, uint32_t bbytes)
+template <class AR, bool AKC, bool BKC, int TS, int BM = TS, int NW = 4>
+__global__ void __launch_bounds__(64 * NW)
+    __attribute__((amdgpu_waves_per_eu(AR::NPLANE > 1 ? 2 : NW == 8 ? 4 : 3)))
+    gdma_gemm(const KParams p, uint32_t abytes, uint32_t bbytes)
 {
-    __shared__ __attribute__((aligned(1024))) char smem[gdma_lds_bytes<AR, TS>()];
+    __shared__ __attribute__((aligned(1024))) char smem[gdma_lds_bytes<AR, TS, BM, NW>()];
     const int tiles_per_inst = p.tiles_m * p.tiles_n;
     for (long t = blockIdx.x; t < p.total; t += gridDim.x) {
         const long w = xcd_remap(t, p.total);
         const int inst = (int)(w / tiles_per_inst);
         const int rem = (int)(w % tiles_per_inst);
-        const TileJob j = make_job(p, inst, (rem % p.tiles_m) * TS, (rem / p.tiles_m) * TS);
-        gdma_job<AR, TS, AKC, BKC>(j, abytes, bbytes, smem, [] {}, [] {});
+        const TileJob j = make_job(p, inst, (rem % p.tiles_m) * BM, (rem / p.tiles_m) * TS);
+        gdma_job<AR, TS, AKC, BKC, BM, NW>(j, abytes, bbytes, smem, [] {}, [] {});
     }
 }
 

@@ -54,7 +54,8 @@ enum TileConfig : int {
     // dimension and alignment; the grouped kernel's tile), three stages
     CFG_GDMA_128 = 24,
     CFG_GDMA_64 = 25,  // the same with 64x64 tiles (four waves of 32x32; other forms: TILE_64)
-    CFG_COUNT = 26
+    CFG_GDMA_256x128 = 26,  // 256x128, 8 waves of 64x64 (other forms: TILE_256x128)
+    CFG_COUNT = 27
 };
 
 const char* config_name(int cfg);

@@ -173,12 +173,12 @@ hipError_t run_direct(KParams p, hipStream_t st)
     return hipGetLastError();
 }
 
-// 128x128 tiles with k-blocks loaded straight into LDS through buffer
-// descriptors (int8, fp32 and its bf16 split forms; any layout, leading
-// dimension and alignment).
+// BM x TS tiles (NW waves) with k-blocks loaded straight into LDS through
+// buffer descriptors (int8, fp32 and its bf16 split forms; any layout,
+// leading dimension and alignment).
 // Other forms, or matrices whose extent does not fit the descriptors' 31-bit
-// offsets: the register-staged 128x128 tile.
-template <class AR, bool AKC, bool BKC, int TS>
+// offsets: the register-staged tile of the same shape.
+template <class AR, bool AKC, bool BKC, int TS, int BM = TS, int NW = 4>
 hipError_t run_gdma(KParams p, hipStream_t st)
 {
     if constexpr (AR::ID == ARITH_I8 || AR::ID == ARITH_F32 || AR::NPLANE > 1) {
@@ -186,16 +186,16 @@ hipError_t run_gdma(KParams p, hipStream_t st)
         const uint32_t abytes = AKC ? wave_extent(p.k, p.m, p.lda, E) : wave_extent(p.m, p.k, p.lda, E);
         const uint32_t bbytes = BKC ? wave_extent(p.k, p.n, p.ldb, E) : wave_extent(p.n, p.k, p.ldb, E);
         if (abytes != 0u && bbytes != 0u) {
-            p.tiles_m = (p.m + TS - 1) / TS;
+            p.tiles_m = (p.m + BM - 1) / BM;
             p.tiles_n = (p.n + TS - 1) / TS;
             p.total = (long)p.batch * p.tiles_m * p.tiles_n;
-            auto kern = gdma_gemm<AR, AKC, BKC, TS>;
-            const long grid = persistent_grid(p.total, kernel_occupancy(kern, 256));
-            hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, st, p, abytes, bbytes);
+            auto kern = gdma_gemm<AR, AKC, BKC, TS, BM, NW>;
+            const long grid = persistent_grid(p.total, kernel_occupancy(kern, 64 * NW));
+            hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * NW), 0, st, p, abytes, bbytes);
             return hipGetLastError();
         }
     }
-    return run_tile<AR, AKC, BKC, TS == 64 ? CFG_TILE_64 : CFG_TILE_128>(p, st);
+    return run_tile<AR, AKC, BKC, BM == 256 ? CFG_TILE_256x128 : TS == 64 ? CFG_TILE_64 : CFG_TILE_128>(p, st);
 }
 
 template <class AR, bool AKC, bool BKC>
@@ -228,6 +228,7 @@ hipError_t run_cfg(const KParams& p, int cfg, hipStream_t st)
     case CFG_DMA_256_K2: return run_dma<AR, AKC, BKC, 256, 256, 128, 2, 4, 2>(p, st);
     case CFG_GDMA_128: return run_gdma<AR, AKC, BKC, 128>(p, st);
     case CFG_GDMA_64: return run_gdma<AR, AKC, BKC, 64>(p, st);
+    case CFG_GDMA_256x128: return run_gdma<AR, AKC, BKC, 128, 256, 8>(p, st);
     default: return hipErrorInvalidValue;
     }
 }

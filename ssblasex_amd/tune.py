@@ -39,7 +39,7 @@ import numpy as np
 CONFIGS = ["direct16", "direct32", "tile64", "tile128", "tile128x64", "tile64x128", "tile128k2", "tile128w8",
            "tile256x128", "tile64s2", "tile128s2", "tile128k2s2", "tile128w8s2", "tile256x128s2", "dma64",
            "dma128", "dma128k2", "dma256x128", "dma128s3", "kc256x128", "kc128", "dma256", "dma256w16", "dma256k2",
-           "gdma128", "gdma64"]
+           "gdma128", "gdma64", "gdma256x128"]
 FORMS = {"i8": 0, "f32": 1, "f64": 2, "bf16x3": 3, "bf16x6": 4}
 COMPUTE = {"i8": 0, "f32": 0, "f64": 0, "bf16x3": 4, "bf16x6": 5}
 TABLE_DIR = Path(__file__).resolve().parent / "tune"

@@ -30,7 +30,7 @@ SHAPES = [(1, 1, 1), (7, 5, 3), (16, 16, 64), (33, 17, 65), (64, 64, 64), (65, 6
 CONFIGS = ["direct16", "direct32", "tile64", "tile128", "tile128x64", "tile64x128", "tile128k2", "tile128w8",
            "tile256x128", "tile64s2", "tile128s2", "tile128k2s2", "tile128w8s2", "tile256x128s2", "dma64",
            "dma128", "dma128k2", "dma256x128", "dma128s3", "kc256x128", "kc128", "dma256", "dma256w16", "dma256k2",
-           "gdma128", "gdma64"]
+           "gdma128", "gdma64", "gdma256x128"]
 
 
 @pytest.fixture(autouse=True)
@@ -209,7 +209,7 @@ SPLIT_FORMS = [("bf16x6", 5, 4.0), ("bf16x3", 4, None)]
 
 
 @pytest.mark.parametrize("cfg", ["tile256x128", "tile256x128s2", "tile128w8", "tile128", "tile64", "gdma128",
-                                 "gdma64"])
+                                 "gdma64", "gdma256x128"])
 @pytest.mark.parametrize("form,ct,factor", SPLIT_FORMS, ids=[f[0] for f in SPLIT_FORMS])
 @pytest.mark.parametrize("ta,tb", OPS)
 def test_split_forms_forced_tiles(cfg, form, ct, factor, ta, tb):
